@@ -1,0 +1,278 @@
+/*
+ * risa_hip.h -- C-ABI of the MI355X reconstruction hot path.
+ *
+ * The drop-in boundary for RISA's per-frame chain
+ *   Attenuation -> Fan2Para -> Filter -> Backprojection -> Masking
+ * (RISA/example/main.cpp:95-134).  The C++ stage classes in
+ * risa_amd/csrc/stages/ (same glados Stage interface as the reference) call
+ * these entry points; a ctypes binding in risa_amd/_lib.py calls them from the
+ * test harness.
+ *
+ * Conventions
+ *  - Every function returns int: RISA_OK (0) or a RISA_ERR_* code; the
+ *    message of the last failure on the calling thread is risa_last_error().
+ *    Nothing throws across this boundary.
+ *  - Device pointers are plain pointers into HBM; `stream` is a hipStream_t
+ *    passed as void* (NULL = the null stream).  Run calls are asynchronous on
+ *    that stream and allocate nothing.
+ *  - A handle is bound to the device it was created on and is
+ *    thread-compatible (one thread at a time), like the reference's
+ *    per-device processor threads.
+ *  - Arrays are row-major: fan sinograms [P_f][D_f], parallel sinograms
+ *    [P][D] (P over 180 degrees), images [N][N] (index x + N*y).
+ */
+#ifndef RISA_HIP_H_
+#define RISA_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RISA_MAX_PLANES 8
+
+enum risa_status {
+    RISA_OK = 0,
+    RISA_ERR_ARG = 1,          /* invalid argument / geometry */
+    RISA_ERR_CONFIG = 2,       /* configuration file missing, unparsable, or key missing */
+    RISA_ERR_HIP = 3,          /* a HIP runtime call failed */
+    RISA_ERR_UNSUPPORTED = 4,  /* geometry outside what a kernel supports */
+    RISA_ERR_ALLOC = 5         /* device or host allocation failed */
+};
+
+enum risa_filter_type { RISA_FILTER_RAMP = 0, RISA_FILTER_SHEPP_LOGAN = 1, RISA_FILTER_COSINE = 2,
+                        RISA_FILTER_HAMMING = 3, RISA_FILTER_HANNING = 4 };
+
+/* Filter execution modes. */
+enum risa_filter_mode {
+    RISA_FILTER_MODE_FFT32 = 0,      /* FP32 batched radix FFT, window fused   */
+    RISA_FILTER_MODE_GEMM_SPLIT = 1  /* DFT-as-GEMM on MFMA, FP16 hi+lo split, FP32 accumulate */
+};
+
+enum risa_dtype { RISA_F32 = 0, RISA_F16 = 1 };
+
+/* The configuration keys the hot path reads (RISA/example/config.cfg), as
+ * libconfig holds them.  numberOfFanProjections is the derived
+ * samplingRate*1000000/scanRate (Attenuation.cu:409, Fan2Para.cu:610).
+ * fusedMasking = 1 makes Backprojection apply the Masking stage's circular
+ * mask itself (pixels outside are never computed). */
+typedef struct risa_geom {
+    int32_t numberOfFanDetectors;
+    int32_t numberOfFanProjections;
+    int32_t numberOfPlanes;
+    int32_t numberOfDetectorModules;
+    int32_t numberOfParallelProjections;
+    int32_t numberOfParallelDetectors;
+    int32_t numberOfPixels;
+    int32_t filterType;            /* enum risa_filter_type */
+    int32_t interpolationType;     /* 0 linear, 1 nearestNeighbour */
+    int32_t useTextureMemory;
+    int32_t normalization;
+    int32_t fusedMasking;
+    int32_t xa, xb, xc, xd, xe, xf;
+    int32_t reserved0;
+    int32_t reserved1;
+    double sourceOffset;
+    double detectorDiameter;
+    double imageCenterX, imageCenterY;
+    double imageWidth;
+    double rotationOffset;
+    double cutoffFraction;
+    double maskingValue;
+    double lowerLimOffset, upperLimOffset;
+    double threshMin, threshMax;
+    double sourceDiameter[RISA_MAX_PLANES];
+    double deltaX[RISA_MAX_PLANES];
+    double deltaZ[RISA_MAX_PLANES];
+    double sourceAngle[RISA_MAX_PLANES];
+} risa_geom;
+
+typedef struct risa_attenuation_s* risa_attenuation_t;
+typedef struct risa_fan2para_s* risa_fan2para_t;
+typedef struct risa_filter_s* risa_filter_t;
+typedef struct risa_bp_s* risa_bp_t;
+typedef struct risa_mask_s* risa_mask_t;
+typedef struct risa_chain_s* risa_chain_t;
+typedef struct risa_bp_batched_s* risa_bp_batched_t;
+
+/* ------------------------------------------------------------------ misc -- */
+const char* risa_last_error(void);
+int risa_abi_version(void);
+size_t risa_geom_size(void);
+
+/* Parse a libconfig-syntax configuration file and fill every key the hot
+ * path reads.  Replaces the per-stage readConfig methods
+ * (Attenuation.cu:383-414, Fan2Para.cu:592-624, Filter.cu:193-219,
+ * Backprojection.cu:210-236, Masking.cu:146-155) over ConfigReader
+ * (ConfigReader.h:41-90).  Missing keys -> RISA_ERR_CONFIG. */
+int risa_geom_from_config(const char* path, risa_geom* out);
+
+/* ------------------------------------------------- host-side init tables -- */
+/* Relevant-area mask [P_f][D_f] (Attenuation::relevantAreaMask,
+ * Attenuation.cu:327-381). */
+int risa_attenuation_area_mask(const risa_geom* g, float* mask);
+
+/* Reference flat-field preparation (Attenuation::init, Attenuation.cu:177-251):
+ * repairs defect detectors in each of the R*planes uint16 reference frames in
+ * place (interpolationFunctions.h:28-97), averages them into
+ * avg_ref [planes][P_f][D_f] and writes the dark average avg_dark
+ * [planes][D_f] (identically 0, as the reference computes it). */
+int risa_attenuation_prepare_reference(const risa_geom* g, uint16_t* ref_frames, int R,
+                                       float* avg_ref, float* avg_dark);
+
+/* Defect detection / repair on one uint16 frame [P][D]
+ * (findDefectDetectors / interpolateDefectDetectors, interpolationFunctions.h:28-97). */
+int risa_find_defect_detectors(const uint16_t* frame, int D, int P, double threshMin,
+                               double threshMax, int32_t* flags);
+int risa_interpolate_defect_detectors(uint16_t* frame, const int32_t* flags, int D, int P);
+
+/* Rebinning tables of one plane in the reference's layout
+ * (Fan2Para::computeFan2ParaTransp, Fan2Para.cu:251-480), P2 = 2P internal
+ * angles: ints = 10 arrays of P2*D {thetaBefore1, thetaAfter1, gammaBefore1,
+ * gammaAfter1, ray1, thetaBefore2, thetaAfter2, gammaBefore2, gammaAfter2,
+ * ray2}; floats = 4 arrays of P2*D {thetaGoal1, thetaGoal2, gammaGoal1,
+ * gammaGoal2}.  Either pointer may be NULL. */
+int risa_fan2para_tables(const risa_geom* g, int plane, int32_t* ints, float* floats);
+
+/* Filter taps h[0..D/2] (Filter::designFilter, Filter.cu:169-191). */
+int risa_filter_taps(const risa_geom* g, float* taps);
+
+/* Backprojection angle table and constants (Backprojection.cu:118-141):
+ * sinT/cosT [P]; consts3 = {scale, normalizationFactor, imageCenter}. */
+int risa_bp_lut(const risa_geom* g, float* sinT, float* cosT, float* consts3);
+
+/* ------------------------------------------------------------ Attenuation -- */
+/* Replaces Attenuation::Attenuation + processor's table upload
+ * (Attenuation.cu:48-79, 118-145).  avg_ref/avg_dark are host arrays. */
+int risa_hip_attenuation_create(const risa_geom* g, const float* avg_ref, const float* avg_dark,
+                                int device, risa_attenuation_t* out);
+/* One frame: computeAttenuation (Attenuation.cu:416-444) launched by
+ * processor (:158-161).  raw [P_f][D_f] uint16 -> out [P_f][D_f] float. */
+int risa_hip_attenuation_run(risa_attenuation_t h, const uint16_t* raw, int plane, float* out,
+                             void* stream);
+/* `frames` consecutive frames; planes[f] is a host array. */
+int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16_t* raw,
+                                   const int32_t* planes, int frames, float* out, void* stream);
+int risa_hip_attenuation_destroy(risa_attenuation_t h);
+
+/* --------------------------------------------------------------- Fan2Para -- */
+/* Interpolation-weight precision of the rebinning table. */
+enum risa_rebin_weights {
+    RISA_REBIN_WEIGHTS_FP64 = 0,  /* fractions evaluated in FP64 on the device (default) */
+    RISA_REBIN_WEIGHTS_FP32 = 1   /* fractions from float goal angles/coordinates, as the reference */
+};
+
+/* Replaces Fan2Para::Fan2Para (table build + upload, Fan2Para.cu:44-149).
+ * Integer decisions (ray flags, brackets) are bit-exact with the reference's
+ * host build; weights per risa_rebin_weights (create = FP64). */
+int risa_hip_fan2para_create(const risa_geom* g, int device, risa_fan2para_t* out);
+int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int weights, risa_fan2para_t* out);
+/* One frame: setValue + interpolation (cuda_kernels_fan2para.h:64-185) as
+ * launched by Fan2Para::processor (Fan2Para.cu:220-238).
+ * fan [P_f][D_f] -> par [P][D]. */
+int risa_hip_fan2para_run(risa_fan2para_t h, const float* fan, int plane, float* par,
+                          void* stream);
+int risa_hip_fan2para_run_batch(risa_fan2para_t h, const float* fan, const int32_t* planes,
+                                int frames, float* par, void* stream);
+/* Device bytes of the packed table (all planes). */
+int risa_hip_fan2para_table_bytes(risa_fan2para_t h, size_t* bytes);
+int risa_hip_fan2para_destroy(risa_fan2para_t h);
+
+/* ----------------------------------------------------------------- Filter -- */
+/* Replaces Filter::Filter + initCuFFT + designFilter (Filter.cu:44-65,
+ * 145-191).  mode: enum risa_filter_mode. */
+int risa_hip_filter_create(const risa_geom* g, int device, int mode, risa_filter_t* out);
+/* Replace the D/2+1 taps (host array); used for learnable taps. */
+int risa_hip_filter_set_taps(risa_filter_t h, const float* taps);
+int risa_hip_filter_get_taps(risa_filter_t h, float* taps);
+/* In place, `slices` sinograms [P][D] (Filter::processor, Filter.cu:102-143). */
+int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, void* stream);
+/* Out of place with FP16 output (value * out_scale stored as half). */
+int risa_hip_filter_run_f16(risa_filter_t h, const float* sino, uint16_t* out, int slices,
+                            float out_scale, void* stream);
+/* Adjoint of the filter (the circulant is symmetric: the same operator). */
+int risa_hip_filter_adjoint_run(risa_filter_t h, float* grad, int slices, void* stream);
+/* Gradient of a loss w.r.t. the D/2+1 taps given the unfiltered input x and
+ * the gradient g w.r.t. the filtered output (both [slices][P][D], device);
+ * dtaps: device [D/2+1], overwritten. */
+int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const float* g, int slices,
+                              float* dtaps, void* stream);
+int risa_hip_filter_destroy(risa_filter_t h);
+
+/* --------------------------------------------------------- Backprojection -- */
+/* Replaces Backprojection::Backprojection + the processor's LUT upload
+ * (Backprojection.cu:46-78, 115-148).  With g->fusedMasking (and no
+ * normalization) the circular mask of Masking.cu:157-169 is applied inside. */
+int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out);
+/* One slice: backProjectLinear / backProjectNearest / backProjectTex
+ * (Backprojection.cu:238-336) as selected by interpolationType and
+ * useTextureMemory.  sino [P][D] f32 -> img [N][N]. */
+int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void* stream);
+/* `slices` sinograms of this geometry stored as FP16 (value * in_scale),
+ * FP32 accumulation; img [slices][N][N].  Linear interpolation only. */
+int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_scale, int slices,
+                        float* img, void* stream);
+/* Adjoint of risa_hip_bp_run (linear): the ray-driven projector that is its
+ * exact transpose (mask included when fused).  img [N][N] -> sino [P][D]. */
+int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream);
+/* Kernel variant: -1 = autotune on first run (default), else an index. */
+int risa_hip_bp_set_variant(risa_bp_t h, int variant);
+int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
+int risa_hip_bp_autotune(risa_bp_t h, void* stream);
+int risa_hip_bp_destroy(risa_bp_t h);
+
+/* ---------------------------------------------------------------- Masking -- */
+/* Masking::Masking / processor (Masking.cu:55-144): optional min-max
+ * normalisation, then the circular mask, in place on img [N][N]. */
+int risa_hip_mask_create(const risa_geom* g, int device, risa_mask_t* out);
+int risa_hip_mask_run(risa_mask_t h, float* img, void* stream);
+int risa_hip_mask_destroy(risa_mask_t h);
+
+/* ------------------------------------------ uneven batched backprojection -- */
+/* One job of a persistent batched backprojection: its own geometry. */
+typedef struct risa_bp_job {
+    const void* sino;       /* [P][D], dtype below */
+    float* img;             /* [N][N] */
+    int32_t N, P, D;
+    int32_t dtype;          /* enum risa_dtype */
+    float in_scale;         /* FP16 sinograms store value * in_scale */
+    float rotationOffset;   /* degrees */
+    int32_t mask;           /* apply the circular mask (fused) */
+    float maskingValue;
+} risa_bp_job;
+
+int risa_hip_bp_batched_create(const risa_bp_job* jobs, int njobs, int device,
+                               risa_bp_batched_t* out);
+/* Re-point the jobs' sino/img (same geometries, same order). */
+int risa_hip_bp_batched_set_buffers(risa_bp_batched_t h, const risa_bp_job* jobs, int njobs);
+int risa_hip_bp_batched_run(risa_bp_batched_t h, void* stream);
+int risa_hip_bp_batched_set_variant(risa_bp_batched_t h, int variant);
+int risa_hip_bp_batched_autotune(risa_bp_batched_t h, void* stream);
+int risa_hip_bp_batched_destroy(risa_bp_batched_t h);
+
+/* ------------------------------------------------------------------ Chain -- */
+/* The whole per-frame chain on one device with its own scratch buffers.
+ * mode 0 = plain FP32 frame by frame (FFT filter, FP32 backprojection);
+ * mode 1 = mixed precision, batched (DFT-GEMM filter with FP16 output,
+ * FP16-sinogram / FP32-accumulate backprojection).  max_frames bounds one
+ * run call. */
+enum risa_chain_mode { RISA_CHAIN_PLAIN = 0, RISA_CHAIN_MIXED = 1 };
+
+int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float* avg_dark,
+                          int device, int mode, int max_frames, risa_chain_t* out);
+/* raw [frames][P_f][D_f] uint16 (device) -> img [frames][N][N] (device);
+ * planes is a host array.  use_graph != 0 replays a captured hipGraph of the
+ * same launch sequence (captured on first use per buffer set). */
+int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
+                       float* img, int use_graph, void* stream);
+/* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
+ * 2 filtered (FP32 modes only). */
+int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
+int risa_hip_chain_destroy(risa_chain_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RISA_HIP_H_ */

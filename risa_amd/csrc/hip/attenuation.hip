@@ -1,0 +1,180 @@
+// Attenuation stage on the device: dark/reference normalisation, -log and the
+// relevant-area mask, one coalesced elementwise pass per frame.
+//
+// Reference: computeAttenuation (Attenuation.cu:416-444), launched per frame
+// by Attenuation::processor (:118-175) with temp = pow(10,-5) (:144).
+//
+// HBM layout: the area mask [P_f][D_f] (shared by planes) is folded into the
+// per-plane reference table refm [planes][P_f][D_f]:
+//   refm = +max(ref - dark, eps) where mask = 1, -max(ref - dark, eps) where mask = 0
+// (the clamped denominator is always positive, so its sign bit is free).
+// A frame then streams 2 B raw + 4 B refm in and 4 B out per element.
+#include "device_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "../host/tables.h"
+
+namespace risa {
+
+struct PlaneList {
+    int32_t p[64];
+};
+
+__global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __restrict__ raw,
+                                                          const float* __restrict__ refm,
+                                                          const float* __restrict__ dark,
+                                                          float* __restrict__ out, int Df, int quads_per_row,
+                                                          int rows, PlaneList planes)
+{
+    const int f = blockIdx.y;
+    const int plane = planes.p[f];
+    const long n = (long)Df * rows;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // one thread: 4 detectors of one row
+    if (q >= (long)quads_per_row * rows) return;
+    const int y = (int)(q / quads_per_row);
+    const int x = (int)(q - (long)y * quads_per_row) * 4;
+    const long e = (long)y * Df + x;
+    const uint16_t* r = raw + (long)f * n;
+    const float* rm = refm + (long)plane * n;
+    const float* dk = dark + (long)plane * Df;
+    float* o = out + (long)f * n;
+    const float temp = 1.0e-5f;   // (float)pow(10,-5)
+
+    const ushort4 v = *reinterpret_cast<const ushort4*>(r + e);
+    const float4 d = *reinterpret_cast<const float4*>(rm + e);
+    const float4 dkv = *reinterpret_cast<const float4*>(dk + x);
+    const float vin[4] = {(float)v.x, (float)v.y, (float)v.z, (float)v.w};
+    const float den[4] = {d.x, d.y, d.z, d.w};
+    const float drk[4] = {dkv.x, dkv.y, dkv.z, dkv.w};
+    float res[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float num = vin[k] - drk[k];
+        if (num < temp) num = temp;
+        const float m = den[k] > 0.0f ? 1.0f : 0.0f;
+        res[k] = -logf(num / fabsf(den[k])) * m;
+    }
+    *reinterpret_cast<float4*>(o + e) = make_float4(res[0], res[1], res[2], res[3]);
+}
+
+// Any D_f: one element per thread.
+__global__ void __launch_bounds__(256) attenuation_kernel_scalar(const uint16_t* __restrict__ raw,
+                                                                 const float* __restrict__ refm,
+                                                                 const float* __restrict__ dark,
+                                                                 float* __restrict__ out, int Df, int rows,
+                                                                 PlaneList planes)
+{
+    const int f = blockIdx.y;
+    const int plane = planes.p[f];
+    const long n = (long)Df * rows;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int x = (int)(e % Df);
+    float num = (float)raw[(long)f * n + e] - dark[(long)plane * Df + x];
+    if (num < 1.0e-5f) num = 1.0e-5f;
+    const float den = refm[(long)plane * n + e];
+    out[(long)f * n + e] = -logf(num / fabsf(den)) * (den > 0.0f ? 1.0f : 0.0f);
+}
+
+}  // namespace risa
+
+struct risa_attenuation_s {
+    int device = 0;
+    risa_geom g{};
+    risa::DeviceBuffer<float> refm;   // [planes][P_f][D_f]
+    risa::DeviceBuffer<float> dark;   // [planes][D_f] (+3 padding for the vector load)
+};
+
+using namespace risa;
+
+extern "C" int risa_hip_attenuation_create(const risa_geom* g, const float* avg_ref, const float* avg_dark,
+                                           int device, risa_attenuation_t* out)
+{
+    return guarded("risa_hip_attenuation_create", [&] {
+        if (!g || !avg_ref || !avg_dark || !out) fail(RISA_ERR_ARG, "null argument");
+        check_geom(g, true, false);
+        DeviceGuard dg(device);
+        auto h = new risa_attenuation_s();
+        h->device = device;
+        h->g = *g;
+        const int Df = g->numberOfFanDetectors, Pf = g->numberOfFanProjections, planes = g->numberOfPlanes;
+        const size_t n = (size_t)Df * Pf;
+        const std::vector<float> mask = area_mask(*g);
+        std::vector<float> refm(n * planes);
+        const float temp = 1.0e-5f;
+        for (int k = 0; k < planes; k++)
+            for (size_t e = 0; e < n; e++) {
+                float den = avg_ref[k * n + e] - avg_dark[k * Df + (e % Df)];
+                if (den < temp) den = temp;
+                refm[k * n + e] = mask[e] != 0.0f ? den : -den;
+            }
+        std::vector<float> dark((size_t)planes * Df + 4, 0.0f);
+        std::copy(avg_dark, avg_dark + (size_t)planes * Df, dark.begin());
+        try {
+            h->refm.resize(refm.size());
+            h->refm.upload(refm);
+            h->dark.resize(dark.size());
+            h->dark.upload(dark);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+static void launch_attenuation(risa_attenuation_t h, const uint16_t* raw, const int32_t* planes, int frames,
+                               float* out, hipStream_t s)
+{
+    const int Df = h->g.numberOfFanDetectors, Pf = h->g.numberOfFanProjections;
+    for (int f0 = 0; f0 < frames; f0 += 64) {
+        const int nf = std::min(64, frames - f0);
+        PlaneList pl{};
+        for (int f = 0; f < nf; f++) {
+            if (planes[f0 + f] < 0 || planes[f0 + f] >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
+            pl.p[f] = planes[f0 + f];
+        }
+        const size_t n = (size_t)Df * Pf;
+        const uint16_t* r = raw + (size_t)f0 * n;
+        float* o = out + (size_t)f0 * n;
+        const bool vec = (Df % 4 == 0) && ((uintptr_t)r % 8 == 0) && ((uintptr_t)o % 16 == 0);
+        if (vec) {
+            const int quads = Df / 4;
+            dim3 grid(ceil_div((long)quads * Pf, 256), nf);
+            hipLaunchKernelGGL(attenuation_kernel, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df, quads,
+                               Pf, pl);
+        } else {
+            dim3 grid(ceil_div((long)n, 256), nf);
+            hipLaunchKernelGGL(attenuation_kernel_scalar, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df,
+                               Pf, pl);
+        }
+        check_launch("attenuation_kernel");
+    }
+}
+
+extern "C" int risa_hip_attenuation_run(risa_attenuation_t h, const uint16_t* raw, int plane, float* out,
+                                        void* stream)
+{
+    return guarded("risa_hip_attenuation_run", [&] {
+        if (!h || !raw || !out) fail(RISA_ERR_ARG, "null argument");
+        const int32_t p = plane;
+        launch_attenuation(h, raw, &p, 1, out, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16_t* raw, const int32_t* planes,
+                                              int frames, float* out, void* stream)
+{
+    return guarded("risa_hip_attenuation_run_batch", [&] {
+        if (!h || !raw || !out || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (frames == 0) return;
+        launch_attenuation(h, raw, planes, frames, out, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_attenuation_destroy(risa_attenuation_t h)
+{
+    return guarded("risa_hip_attenuation_destroy", [&] { delete h; });
+}

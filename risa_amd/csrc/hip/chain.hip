@@ -1,0 +1,161 @@
+// The whole per-frame chain on one device:
+//   Attenuation -> Fan2Para -> Filter -> Backprojection (+ fused Masking)
+// as RISA/example/main.cpp:95-134 wires the stages, without the per-stage
+// host synchronisation of the reference (each stage there syncs its stream
+// before handing the frame on, e.g. Backprojection.cu:203): launches are
+// stream-ordered, and a captured hipGraph replays the launch sequence.
+//
+// PLAIN: frame by frame, FP32 throughout (FFT filter, FP32 backprojector).
+// MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA with FP16
+//        output, FP16-sinogram / FP32-accumulate backprojection.
+#include "device_common.h"
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows, hipStream_t s);
+int risa_bp_run_batch_f32(risa_bp_t h, const float* sino, float* img, int slices, hipStream_t s);
+
+namespace {
+constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
+}
+
+struct risa_chain_s {
+    int device = 0;
+    int mode = 0;
+    int max_frames = 0;
+    risa_geom g{};
+    risa_attenuation_t att = nullptr;
+    risa_fan2para_t f2p = nullptr;
+    risa_filter_t filt = nullptr;
+    risa_bp_t bp = nullptr;
+    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;
+    risa::DeviceBuffer<uint16_t> f16_buf;
+    std::map<std::tuple<const void*, const void*, int, std::vector<int32_t>>, hipGraphExec_t> graphs;
+    ~risa_chain_s()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        if (bp) risa_hip_bp_destroy(bp);
+        if (filt) risa_hip_filter_destroy(filt);
+        if (f2p) risa_hip_fan2para_destroy(f2p);
+        if (att) risa_hip_attenuation_destroy(att);
+    }
+};
+
+using namespace risa;
+
+static void ok(int rc)
+{
+    if (rc != RISA_OK) fail(rc, "%s", risa_last_error());
+}
+
+extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float* avg_dark, int device,
+                                     int mode, int max_frames, risa_chain_t* out)
+{
+    return guarded("risa_hip_chain_create", [&] {
+        if (!g || !avg_ref || !avg_dark || !out || max_frames <= 0) fail(RISA_ERR_ARG, "bad argument");
+        if (mode != RISA_CHAIN_PLAIN && mode != RISA_CHAIN_MIXED) fail(RISA_ERR_ARG, "unknown chain mode");
+        DeviceGuard dg(device);
+        auto h = new risa_chain_s();
+        try {
+            h->device = device;
+            h->mode = mode;
+            h->max_frames = max_frames;
+            h->g = *g;
+            ok(risa_hip_attenuation_create(g, avg_ref, avg_dark, device, &h->att));
+            ok(risa_hip_fan2para_create(g, device, &h->f2p));
+            ok(risa_hip_filter_create(g, device, mode == RISA_CHAIN_MIXED ? RISA_FILTER_MODE_GEMM_SPLIT : RISA_FILTER_MODE_FFT32,
+                                      &h->filt));
+            ok(risa_hip_bp_create(g, device, &h->bp));
+            const size_t fan = (size_t)g->numberOfFanDetectors * g->numberOfFanProjections;
+            const size_t par = (size_t)g->numberOfParallelDetectors * g->numberOfParallelProjections;
+            const int slots = mode == RISA_CHAIN_PLAIN ? 1 : max_frames;
+            h->att_buf.resize(fan * slots);
+            h->par_buf.resize(par * slots);
+            if (mode == RISA_CHAIN_MIXED) h->f16_buf.resize(par * slots);
+            else h->filt_buf.resize(par);
+            ok(risa_hip_bp_autotune(h->bp, nullptr));
+            RISA_HIP_CHECK(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img, hipStream_t s)
+{
+    const risa_geom& g = h->g;
+    const size_t fan = (size_t)g.numberOfFanDetectors * g.numberOfFanProjections;
+    const size_t par = (size_t)g.numberOfParallelDetectors * g.numberOfParallelProjections;
+    const size_t out = (size_t)g.numberOfPixels * g.numberOfPixels;
+    if (h->mode == RISA_CHAIN_PLAIN) {
+        for (int f = 0; f < frames; f++) {
+            ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], h->att_buf.ptr, s));
+            ok(risa_hip_fan2para_run(h->f2p, h->att_buf.ptr, planes[f], h->par_buf.ptr, s));
+            ok(risa_filter_run_oop(h->filt, h->par_buf.ptr, h->filt_buf.ptr, g.numberOfParallelProjections, s));
+            ok(risa_hip_bp_run(h->bp, h->filt_buf.ptr, img + f * out, s));
+        }
+        return;
+    }
+    ok(risa_hip_attenuation_run_batch(h->att, raw, planes, frames, h->att_buf.ptr, s));
+    ok(risa_hip_fan2para_run_batch(h->f2p, h->att_buf.ptr, planes, frames, h->par_buf.ptr, s));
+    ok(risa_hip_filter_run_f16(h->filt, h->par_buf.ptr, h->f16_buf.ptr, frames, kF16SinoScale, s));
+    ok(risa_hip_bp_run_f16(h->bp, h->f16_buf.ptr, kF16SinoScale, frames, img, s));
+    (void)par;
+}
+
+extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img,
+                                  int use_graph, void* stream)
+{
+    return guarded("risa_hip_chain_run", [&] {
+        if (!h || !raw || !planes || !img || frames < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (frames > h->max_frames) fail(RISA_ERR_ARG, "frames (%d) exceeds max_frames (%d)", frames, h->max_frames);
+        if (frames == 0) return;
+        hipStream_t s = as_stream(stream);
+        if (!use_graph) {
+            enqueue(h, raw, planes, frames, img, s);
+            return;
+        }
+        if (!s) fail(RISA_ERR_ARG, "graph replay needs a non-null stream");
+        auto key = std::make_tuple((const void*)raw, (const void*)img, frames, std::vector<int32_t>(planes, planes + frames));
+        auto it = h->graphs.find(key);
+        if (it == h->graphs.end()) {
+            hipGraph_t graph;
+            RISA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(h, raw, planes, frames, img, s);
+            } catch (...) {
+                hipGraph_t dummy;
+                (void)hipStreamEndCapture(s, &dummy);
+                throw;
+            }
+            RISA_HIP_CHECK(hipStreamEndCapture(s, &graph));
+            hipGraphExec_t exec;
+            RISA_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            it = h->graphs.emplace(key, exec).first;
+        }
+        RISA_HIP_CHECK(hipGraphLaunch(it->second, s));
+    });
+}
+
+extern "C" int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr)
+{
+    return guarded("risa_hip_chain_intermediate", [&] {
+        if (!h || !ptr) fail(RISA_ERR_ARG, "null argument");
+        switch (which) {
+            case 0: *ptr = h->att_buf.ptr; break;
+            case 1: *ptr = h->par_buf.ptr; break;
+            case 2: *ptr = h->mode == RISA_CHAIN_PLAIN ? (const void*)h->filt_buf.ptr : (const void*)h->f16_buf.ptr; break;
+            default: fail(RISA_ERR_ARG, "which must be 0, 1 or 2");
+        }
+    });
+}
+
+extern "C" int risa_hip_chain_destroy(risa_chain_t h)
+{
+    return guarded("risa_hip_chain_destroy", [&] { delete h; });
+}

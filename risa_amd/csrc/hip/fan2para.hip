@@ -1,0 +1,273 @@
+// Fan2Para stage on the device: fan -> parallel rebinning as a gather over a
+// packed per-output table, with the reference's 360 -> 180 degree fold done
+// as a deterministic two-term sum (no zero fill, no atomics).
+//
+// Reference: interpolation kernel (cuda_kernels_fan2para.h:64-177), setValue
+// (:179-185), launched by Fan2Para::processor (Fan2Para.cu:189-249) over the
+// 14 host-built arrays of Fan2Para::computeFan2ParaTransp (:251-480).
+//
+// Output (j, i), j < P, receives half of internal sample A = (j, i) and half
+// of internal sample B = (j + P, D-1-i) -- the reference's two atomicAdd's
+// onto a zeroed buffer.  Each sample mixes up to two rays; each ray is a
+// bilinear interpolation of the fan sinogram in (theta, gamma).  The table
+// holds per output four ray slots of {packed indices, f_theta, f_gamma}:
+// 48 B/output instead of the reference's 2 x 56 B.
+#include "device_common.h"
+
+#include <algorithm>
+
+#include "../host/tables.h"
+
+namespace risa {
+
+struct PlaneList2 {
+    int32_t p[64];
+};
+
+__device__ __forceinline__ float rebin_ray(const float* __restrict__ fan, uint32_t idx, float ft, float fg, int Df,
+                                           int Pf)
+{
+    const int tb = (int)(idx & 0xFFFFu);
+    const int gb = (int)((idx >> 16) & 0x7FFFu);
+    const int ta = (tb + 1 == Pf) ? 0 : tb + 1;
+    const int ga = (gb + 1 == Df) ? 0 : gb + 1;
+    const float W1 = fan[tb * Df + gb];
+    const float W2 = fan[tb * Df + ga];
+    const float W3 = fan[ta * Df + gb];
+    const float W4 = fan[ta * Df + ga];
+    const float V1 = W1 + ft * (W3 - W1);
+    const float V2 = W2 + ft * (W4 - W2);
+    return V1 + fg * (V2 - V1);
+}
+
+// Flag-weighted mean of the two rays of one internal sample (:153-158).
+__device__ __forceinline__ float rebin_sample(const float* __restrict__ fan, uint32_t i1, uint32_t i2, float ft1,
+                                              float ft2, float fg1, float fg2, int Df, int Pf)
+{
+    const bool r1 = (i1 >> 31) != 0, r2 = (i2 >> 31) != 0;
+    const float z1 = r1 ? rebin_ray(fan, i1, ft1, fg1, Df, Pf) : 0.0f;
+    const float z2 = r2 ? rebin_ray(fan, i2, ft2, fg2, Df, Pf) : 0.0f;
+    if (r1 && r2) return 0.5f * z1 + 0.5f * z2;
+    return r1 ? z1 : z2;
+}
+
+__global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fan, const uint4* __restrict__ table,
+                                                    float* __restrict__ par, int Df, int Pf, int D, int P,
+                                                    long table_plane_stride, PlaneList2 planes)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int f = blockIdx.z;
+    if (i >= D) return;
+    const long e = (long)j * D + i;
+    const uint4* q = table + (long)planes.p[f] * table_plane_stride + e * 3;
+    const uint4 idx = q[0];
+    const uint4 ftb = q[1];
+    const uint4 fgb = q[2];
+    const float* fr = fan + (long)f * Df * Pf;
+    const float zA = rebin_sample(fr, idx.x, idx.y, __uint_as_float(ftb.x), __uint_as_float(ftb.y),
+                                  __uint_as_float(fgb.x), __uint_as_float(fgb.y), Df, Pf);
+    const float zB = rebin_sample(fr, idx.z, idx.w, __uint_as_float(ftb.z), __uint_as_float(ftb.w),
+                                  __uint_as_float(fgb.z), __uint_as_float(fgb.w), Df, Pf);
+    par[(long)f * D * P + e] = 0.5f * zA + 0.5f * zB;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation weights in FP64 (built once per geometry).
+//
+// The integer decisions of the table (ray flags, theta/gamma brackets, the
+// 2 pi wraps) come from the host build, bit-exact with the reference.  The
+// two interpolation fractions per ray are then re-evaluated here in double
+// from the same formulas (Fan2Para.cu:286-309, :354-480, the ellipse map of
+// cuda_kernels_fan2para.h:38-62) and rounded to float once, instead of being
+// formed from float goal angles and float coordinates.  The host leaves each
+// ray's float goal angles in the weight slots; the double goal angle is taken
+// at the 2 pi representative nearest to them, so wrap decisions stay the
+// host's.
+// ---------------------------------------------------------------------------
+struct RebinGeo {
+    int P, P2, D, Pf, Df;
+    double sourceOffset, rD, rT, dX, dZ, W, L, kappa;
+};
+
+__device__ double ring_angle64(double alpha, double DX, double DZ, double diam)
+{
+    const double L = sqrt(DX * DX + DZ * DZ), R = 0.5 * diam, CA = cos(alpha);
+    const double e0 = fmin(1.0, fmax(-1.0, (L * L + R * DX * CA) / (L * sqrt(L * L + R * R + 2.0 * R * DX * CA))));
+    const double eps = acos(e0);
+    const double p1 = (L * L - R * DX) / (L * sqrt(L * L + R * R - 2.0 * R * DX));
+    const double p2 = (L * L + R * DX) / (L * sqrt(L * L + R * R + 2.0 * R * DX));
+    const double gam = 0.5 * (acos(p1) - acos(p2));
+    const double den = sqrt(eps * eps + 2.0 * eps * gam * CA + gam * gam);
+    const double ae = den > 0.0 ? fmin(1.0, fmax(-1.0, (eps * CA + gam) / den)) : 1.0;
+    return alpha <= M_PI ? acos(ae) : 2.0 * M_PI - acos(ae);
+}
+
+__global__ void rebin_weights_f64_kernel(uint32_t* __restrict__ q, RebinGeo g)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)g.P * g.D * 4) return;
+    const long out = e / 4;
+    const int slot = (int)(e % 4);
+    const int j = (int)(out / g.D), i = (int)(out % g.D);
+    uint32_t* w = q + out * 12;
+    const uint32_t idx = w[slot];
+    if (!(idx >> 31)) return;
+    const int jj = slot < 2 ? j : j + g.P;
+    const int ii = slot < 2 ? i : g.D - 1 - i;
+    const bool ray1 = (slot % 2) == 0;
+    const double twoPi = 2.0 * M_PI;
+    double alpha = 0.0;   // alpha[0] stays 0 in the reference table
+    if (jj > 0) {
+        alpha = (2 * M_PI / 360.0) * (jj * (360.0 / g.P2)) + M_PI / 2;
+        if (alpha > twoPi) alpha -= twoPi;
+    }
+    const double s = -0.5 * g.W + ((0.5 + ii) * g.W) / g.D;
+    const double off = s - g.L * sin(alpha - g.kappa);
+    const double t1 = asin(off / g.rT), eps = asin(off / g.rD);
+    double tg, gg;
+    if (ray1) {
+        tg = alpha - t1;
+        if (tg < 0) tg += twoPi;
+        tg = ring_angle64(tg, g.dX, g.dZ, 2 * g.rT);
+        gg = eps + alpha - 1.5 * M_PI;
+        if (gg < 0) gg += twoPi;
+        if (gg > twoPi) gg -= twoPi;
+    } else {
+        tg = alpha + t1 - M_PI;
+        if (tg < 0) tg += twoPi;
+        tg = ring_angle64(tg, g.dX, g.dZ, 2 * g.rT);
+        gg = -eps + alpha - M_PI / 2.0;
+        if (gg < 0) gg += twoPi;
+    }
+    const double tg32 = (double)__uint_as_float(w[4 + slot]), gg32 = (double)__uint_as_float(w[8 + slot]);
+    tg += twoPi * rint((tg32 - tg) / twoPi);
+    gg += twoPi * rint((gg32 - gg) / twoPi);
+    const int tb = (int)(idx & 0xFFFFu), gb = (int)((idx >> 16) & 0x7FFFu);
+    const int ta = (tb + 1 == g.Pf) ? 0 : tb + 1, ga = (gb + 1 == g.Df) ? 0 : gb + 1;
+    auto theta = [&](int x) {
+        double v = x * (360.0 / g.Pf) - g.sourceOffset;
+        if (v < 0.0) v += 360.0;
+        return (2 * M_PI / 360.0) * v;
+    };
+    auto gamma = [&](int x) { return (2 * M_PI / 360.0) * (x * (360.0 / g.Df)); };
+    const double ft = (tg - theta(tb)) / (theta(ta) - theta(tb));
+    const double fg = (gg - gamma(gb)) / (gamma(ga) - gamma(gb));
+    w[4 + slot] = __float_as_uint((float)ft);
+    w[8 + slot] = __float_as_uint((float)fg);
+}
+
+}  // namespace risa
+
+struct risa_fan2para_s {
+    int device = 0;
+    risa_geom g{};
+    risa::DeviceBuffer<uint32_t> table;   // [planes][P][D][12]
+    size_t plane_words = 0;
+};
+
+using namespace risa;
+
+extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int weights, risa_fan2para_t* out)
+{
+    return guarded("risa_hip_fan2para_create_ex", [&] {
+        if (!g || !out) fail(RISA_ERR_ARG, "null argument");
+        if (weights != RISA_REBIN_WEIGHTS_FP64 && weights != RISA_REBIN_WEIGHTS_FP32)
+            fail(RISA_ERR_ARG, "unknown weights mode");
+        check_geom(g, true, true);
+        DeviceGuard dg(device);
+        auto h = new risa_fan2para_s();
+        try {
+            h->device = device;
+            h->g = *g;
+            const int P = g->numberOfParallelProjections, D = g->numberOfParallelDetectors;
+            h->plane_words = (size_t)P * D * 12;
+            h->table.resize(h->plane_words * g->numberOfPlanes);
+            for (int k = 0; k < g->numberOfPlanes; k++) {
+                const RebinTables t = rebin_tables(*g, k);
+                const PackedRebin pk = pack_rebin(*g, t, weights == RISA_REBIN_WEIGHTS_FP64);
+                uint32_t* dst = h->table.ptr + k * h->plane_words;
+                RISA_HIP_CHECK(hipMemcpy(dst, pk.q.data(), pk.q.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+                if (weights == RISA_REBIN_WEIGHTS_FP64) {
+                    RebinGeo rg{};
+                    rg.P = P; rg.P2 = 2 * P; rg.D = D;
+                    rg.Pf = g->numberOfFanProjections; rg.Df = g->numberOfFanDetectors;
+                    rg.sourceOffset = g->sourceOffset;
+                    rg.rD = g->detectorDiameter / 2.0;
+                    rg.rT = g->sourceDiameter[k] / 2.0;
+                    rg.dX = g->deltaX[k]; rg.dZ = g->deltaZ[k];
+                    rg.W = g->imageWidth;
+                    const double cx = g->imageCenterX, cy = g->imageCenterY;
+                    rg.L = std::sqrt(cx * cx + cy * cy);
+                    rg.kappa = cy != 0 ? std::atan(cx / cy) + (cy < 0 ? M_PI : 0.0) : (cx < 0 ? -M_PI / 2 : M_PI / 2);
+                    if (cx == 0 && cy == 0) rg.kappa = 0.0;
+                    const long n = (long)P * D * 4;
+                    hipLaunchKernelGGL(rebin_weights_f64_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, nullptr, dst, rg);
+                    check_launch("rebin_weights_f64_kernel");
+                    RISA_HIP_CHECK(hipDeviceSynchronize());
+                }
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+extern "C" int risa_hip_fan2para_create(const risa_geom* g, int device, risa_fan2para_t* out)
+{
+    return risa_hip_fan2para_create_ex(g, device, RISA_REBIN_WEIGHTS_FP64, out);
+}
+
+static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* planes, int frames, float* par,
+                         hipStream_t s)
+{
+    const int Df = h->g.numberOfFanDetectors, Pf = h->g.numberOfFanProjections;
+    const int D = h->g.numberOfParallelDetectors, P = h->g.numberOfParallelProjections;
+    for (int f0 = 0; f0 < frames; f0 += 64) {
+        const int nf = std::min(64, frames - f0);
+        PlaneList2 pl{};
+        for (int f = 0; f < nf; f++) {
+            if (planes[f0 + f] < 0 || planes[f0 + f] >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
+            pl.p[f] = planes[f0 + f];
+        }
+        const int bx = D >= 256 ? 256 : 64;
+        dim3 grid(ceil_div(D, bx), P, nf);
+        hipLaunchKernelGGL(rebin_kernel, grid, dim3(bx), 0, s, fan + (size_t)f0 * Df * Pf,
+                           reinterpret_cast<const uint4*>(h->table.ptr), par + (size_t)f0 * D * P, Df, Pf, D, P,
+                           (long)(h->plane_words / 4), pl);
+        check_launch("rebin_kernel");
+    }
+}
+
+extern "C" int risa_hip_fan2para_run(risa_fan2para_t h, const float* fan, int plane, float* par, void* stream)
+{
+    return guarded("risa_hip_fan2para_run", [&] {
+        if (!h || !fan || !par) fail(RISA_ERR_ARG, "null argument");
+        const int32_t p = plane;
+        launch_rebin(h, fan, &p, 1, par, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_fan2para_run_batch(risa_fan2para_t h, const float* fan, const int32_t* planes, int frames,
+                                           float* par, void* stream)
+{
+    return guarded("risa_hip_fan2para_run_batch", [&] {
+        if (!h || !fan || !par || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (frames) launch_rebin(h, fan, planes, frames, par, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_fan2para_table_bytes(risa_fan2para_t h, size_t* bytes)
+{
+    return guarded("risa_hip_fan2para_table_bytes", [&] {
+        if (!h || !bytes) fail(RISA_ERR_ARG, "null argument");
+        *bytes = h->table.count * sizeof(uint32_t);
+    });
+}
+
+extern "C" int risa_hip_fan2para_destroy(risa_fan2para_t h)
+{
+    return guarded("risa_hip_fan2para_destroy", [&] { delete h; });
+}

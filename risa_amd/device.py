@@ -1,0 +1,245 @@
+"""Device wrappers over the C-ABI for torch tensors (test and bench harness).
+
+Every call goes to librisa_hip.so on the tensor's device and torch's current
+stream; there is no CPU path here.  Handles are released by close() or
+when the wrapper is collected.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import BpJob, Geom, check, load, ptr
+
+
+def _stream():
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(device) -> int:
+    import torch
+
+    d = torch.device(device) if not isinstance(device, int) else torch.device("cuda", device)
+    return 0 if d.index is None else d.index
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self.h = C.c_void_p()
+
+    def close(self):
+        if self.h and self.h.value:
+            check(getattr(load(), self._destroy)(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise ValueError("device tensors are required")
+
+
+class Attenuation(_Handle):
+    _destroy = "risa_hip_attenuation_destroy"
+
+    def __init__(self, g: Geom, avg_ref: np.ndarray, avg_dark: np.ndarray, device=0):
+        super().__init__()
+        self.g = g
+        avg_ref = np.ascontiguousarray(avg_ref, dtype=np.float32)
+        avg_dark = np.ascontiguousarray(avg_dark, dtype=np.float32)
+        check(load().risa_hip_attenuation_create(C.byref(g), ptr(avg_ref), ptr(avg_dark), _dev(device), C.byref(self.h)))
+
+    def __call__(self, raw, plane: int, out):
+        _require_cuda(raw, out)
+        check(load().risa_hip_attenuation_run(self.h, ptr(raw), plane, ptr(out), _stream()))
+        return out
+
+    def batch(self, raw, planes, out):
+        _require_cuda(raw, out)
+        pl = np.ascontiguousarray(planes, dtype=np.int32)
+        check(load().risa_hip_attenuation_run_batch(self.h, ptr(raw), ptr(pl), len(pl), ptr(out), _stream()))
+        return out
+
+
+class Fan2Para(_Handle):
+    _destroy = "risa_hip_fan2para_destroy"
+
+    def __init__(self, g: Geom, device=0, weights: int = 0):
+        """weights: 0 = FP64 interpolation fractions (default), 1 = FP32 as the reference."""
+        super().__init__()
+        self.g = g
+        check(load().risa_hip_fan2para_create_ex(C.byref(g), _dev(device), weights, C.byref(self.h)))
+
+    def __call__(self, fan, plane: int, out):
+        _require_cuda(fan, out)
+        check(load().risa_hip_fan2para_run(self.h, ptr(fan), plane, ptr(out), _stream()))
+        return out
+
+    def batch(self, fan, planes, out):
+        _require_cuda(fan, out)
+        pl = np.ascontiguousarray(planes, dtype=np.int32)
+        check(load().risa_hip_fan2para_run_batch(self.h, ptr(fan), ptr(pl), len(pl), ptr(out), _stream()))
+        return out
+
+    def table_bytes(self) -> int:
+        n = C.c_size_t()
+        check(load().risa_hip_fan2para_table_bytes(self.h, C.byref(n)))
+        return n.value
+
+
+class Filter(_Handle):
+    _destroy = "risa_hip_filter_destroy"
+
+    def __init__(self, g: Geom, mode: int = 0, device=0):
+        super().__init__()
+        self.g = g
+        self.mode = mode
+        check(load().risa_hip_filter_create(C.byref(g), _dev(device), mode, C.byref(self.h)))
+
+    def __call__(self, sino, slices: int = 1):
+        _require_cuda(sino)
+        check(load().risa_hip_filter_run(self.h, ptr(sino), slices, _stream()))
+        return sino
+
+    def f16(self, sino, out16, slices: int, out_scale: float):
+        _require_cuda(sino, out16)
+        check(load().risa_hip_filter_run_f16(self.h, ptr(sino), ptr(out16), slices, out_scale, _stream()))
+        return out16
+
+    def adjoint(self, grad, slices: int = 1):
+        _require_cuda(grad)
+        check(load().risa_hip_filter_adjoint_run(self.h, ptr(grad), slices, _stream()))
+        return grad
+
+    def grad_taps(self, x, g, slices: int, dtaps):
+        _require_cuda(x, g, dtaps)
+        check(load().risa_hip_filter_grad_taps(self.h, ptr(x), ptr(g), slices, ptr(dtaps), _stream()))
+        return dtaps
+
+    def set_taps(self, taps: np.ndarray):
+        t = np.ascontiguousarray(taps, dtype=np.float32)
+        check(load().risa_hip_filter_set_taps(self.h, ptr(t)))
+
+    def taps(self) -> np.ndarray:
+        t = np.empty(self.g.numberOfParallelDetectors // 2 + 1, dtype=np.float32)
+        check(load().risa_hip_filter_get_taps(self.h, ptr(t)))
+        return t
+
+
+class Backprojection(_Handle):
+    _destroy = "risa_hip_bp_destroy"
+
+    def __init__(self, g: Geom, device=0):
+        super().__init__()
+        self.g = g
+        check(load().risa_hip_bp_create(C.byref(g), _dev(device), C.byref(self.h)))
+
+    def __call__(self, sino, img):
+        _require_cuda(sino, img)
+        check(load().risa_hip_bp_run(self.h, ptr(sino), ptr(img), _stream()))
+        return img
+
+    def f16(self, sino16, in_scale: float, slices: int, img):
+        _require_cuda(sino16, img)
+        check(load().risa_hip_bp_run_f16(self.h, ptr(sino16), in_scale, slices, ptr(img), _stream()))
+        return img
+
+    def adjoint(self, img, sino):
+        _require_cuda(img, sino)
+        check(load().risa_hip_bp_adjoint_run(self.h, ptr(img), ptr(sino), _stream()))
+        return sino
+
+    def autotune(self):
+        check(load().risa_hip_bp_autotune(self.h, _stream()))
+
+    @property
+    def variant(self) -> int:
+        v = C.c_int()
+        check(load().risa_hip_bp_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    @variant.setter
+    def variant(self, v: int):
+        check(load().risa_hip_bp_set_variant(self.h, v))
+
+
+class Masking(_Handle):
+    _destroy = "risa_hip_mask_destroy"
+
+    def __init__(self, g: Geom, device=0):
+        super().__init__()
+        check(load().risa_hip_mask_create(C.byref(g), _dev(device), C.byref(self.h)))
+
+    def __call__(self, img):
+        _require_cuda(img)
+        check(load().risa_hip_mask_run(self.h, ptr(img), _stream()))
+        return img
+
+
+class BatchedBackprojection(_Handle):
+    """One persistent launch over jobs of different geometry."""
+
+    _destroy = "risa_hip_bp_batched_destroy"
+
+    def __init__(self, jobs, device=0):
+        super().__init__()
+        self.jobs = jobs
+        arr = (BpJob * len(jobs))(*[self._job(**j) for j in jobs])
+        check(load().risa_hip_bp_batched_create(arr, len(jobs), _dev(device), C.byref(self.h)))
+
+    @staticmethod
+    def _job(sino, img, N, P, D, dtype=0, in_scale=1.0, rotationOffset=0.0, mask=1, maskingValue=0.0):
+        return BpJob(ptr(sino), ptr(img), N, P, D, dtype, in_scale, rotationOffset, mask, maskingValue)
+
+    def __call__(self):
+        check(load().risa_hip_bp_batched_run(self.h, _stream()))
+
+    def autotune(self):
+        check(load().risa_hip_bp_batched_autotune(self.h, _stream()))
+
+    def set_variant(self, v: int):
+        check(load().risa_hip_bp_batched_set_variant(self.h, v))
+
+
+class Chain(_Handle):
+    """The whole per-frame chain (plain FP32 or mixed precision)."""
+
+    _destroy = "risa_hip_chain_destroy"
+
+    def __init__(self, g: Geom, avg_ref: np.ndarray, avg_dark: np.ndarray, mode: int = 0, max_frames: int = 8,
+                 device=0):
+        super().__init__()
+        self.g = g
+        avg_ref = np.ascontiguousarray(avg_ref, dtype=np.float32)
+        avg_dark = np.ascontiguousarray(avg_dark, dtype=np.float32)
+        check(load().risa_hip_chain_create(C.byref(g), ptr(avg_ref), ptr(avg_dark), _dev(device), mode, max_frames,
+                                           C.byref(self.h)))
+
+    def __call__(self, raw, planes, img, use_graph: bool = False):
+        _require_cuda(raw, img)
+        pl = np.ascontiguousarray(planes, dtype=np.int32)
+        check(load().risa_hip_chain_run(self.h, ptr(raw), ptr(pl), len(pl), ptr(img), 1 if use_graph else 0,
+                                        _stream()))
+        return img
+
+    def intermediate(self, which: int) -> int:
+        p = C.c_void_p()
+        check(load().risa_hip_chain_intermediate(self.h, which, C.byref(p)))
+        return p.value
