@@ -36,6 +36,7 @@ struct BpParams {
     int tilesX;
     float invW;
     float out_scale;     // multiplies sum * norm (1/in_scale for FP16 input)
+    int AE;              // angles per staged chunk
 };
 
 // Detector coordinate of a pixel; the same operation sequence everywhere
@@ -53,20 +54,82 @@ __device__ __forceinline__ bool bp_outside(int x, int y, int N, float center)
     return (double)dist > (double)N * N * 0.25;
 }
 
-// One TILE x TILE tile of one slice (the body shared by the per-slice kernel
-// and the persistent uneven-batch kernel).  All threads of the workgroup
-// call it together.  in_f16: the sinogram is FP16 (value * in_scale).
-template <int TILE, int PPT, int A>
+// floor(t) as an int in one instruction (v_cvt_flr_i32_f32); the compiler
+// only selects it under finite-math assumptions, which this file does not
+// make globally.
+__device__ __forceinline__ int floor_to_int(float t)
+{
+    int k;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(k) : "v"(t));
+    return k;
+}
+
+// Splits t into the sample index floor(t) and the fraction t - floor(t)
+// (the adjoint projector's split).
+__device__ __forceinline__ void bp_split(float t, int& k, float& f)
+{
+    k = floor_to_int(t);
+    f = __builtin_amdgcn_fractf(t);
+}
+
+// Per-tile preparation of one angle.
+//
+// t relative to an integer origin T (the floor of the tile's smallest t):
+//   t' = fma(yp, sin, fma(xp, cos, -T)),
+// monotone in x and y under IEEE rounding, so the tile corners bound it and
+// the window of sample indices k' = floor(t') is [kmin', kmin' + W).  The
+// absolute detector index of k' is k' + T + cI.
+struct AngleWin {
+    float c, s, negT;
+    int kmin;   // kmin'
+    int kabs;   // absolute detector index of kmin'
+};
+
+__device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, float xb, float ya, float yb, int cI)
+{
+    const float tmin = fminf(fminf(bp_t(xa, ya, c, s), bp_t(xb, ya, c, s)),
+                             fminf(bp_t(xa, yb, c, s), bp_t(xb, yb, c, s)));
+    const float T = floorf(tmin);
+    const float ua = __builtin_fmaf(xa, c, -T), ub = __builtin_fmaf(xb, c, -T);
+    const float tm = fminf(fminf(__builtin_fmaf(ya, s, ua), __builtin_fmaf(ya, s, ub)),
+                           fminf(__builtin_fmaf(yb, s, ua), __builtin_fmaf(yb, s, ub)));
+    AngleWin w;
+    w.c = c;
+    w.s = s;
+    w.negT = -T;
+    w.kmin = floor_to_int(tm);
+    w.kabs = w.kmin + (int)T + cI;
+    return w;
+}
+
+// LDS layout of one tile: two row buffers of AE angles x W entries first
+// (constant offset, so row addresses fold into the ds_read immediate), then
+// info[P] = (cos, sin, -T, byte offset of k' = 0 of the angle's row) and
+// kinfo[P] = (kmin', kabs).
+//
+// Staged entry of sample k' (w = k' - kmin'):  (e0, d) with
+//   d = s[k+1] - s[k],  e0 = s[k] - k' d,   k = k' + T + cI
+// (zeros outside [0, D)), so that s[k] + (t' - k') d = e0 + t' d and the
+// update needs no fraction: acc += e0; acc = fma(t', d, acc).
+//
+// One TILE x TILE tile of one slice, shared by the per-slice kernel and the
+// persistent uneven-batch kernel.  All threads of the workgroup call it
+// together.  in_f16: the sinogram is FP16 (value * in_scale).
+template <int TX, int TY, int PPT, int E, int U>
 __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* __restrict__ I,
                                         const float* __restrict__ cosT, const float* __restrict__ sinT,
-                                        const BpParams& p, int x0, int y0, int in_f16, float2* rows, float4* info)
+                                        const BpParams& p, int x0, int y0, int in_f16, char* smem)
 {
-    constexpr int NT = TILE * TILE / PPT;
-    constexpr int RS = TILE / PPT;
+    constexpr int NT = TX * TY / PPT;
+    constexpr int RS = TY / PPT;
     const int tid = threadIdx.x;
-    const int tx = tid % TILE, ty = tid / TILE;
+    const int tx = tid % TX, ty = tid / TX;
     const float* S = reinterpret_cast<const float*>(sino_v);
     const uint16_t* S16 = reinterpret_cast<const uint16_t*>(sino_v);
+    const int W = p.W, AE = p.AE;
+    float2* rows = reinterpret_cast<float2*>(smem);
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(float2));
+    int2* kinfo = reinterpret_cast<int2*>(info + p.P);
 
     const int x = x0 + tx;
     const float xp = ((float)x - p.center) * p.scale;
@@ -94,82 +157,108 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
     for (int r = 0; r < PPT; r++) acc[r] = 0.0f;
 
-    const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TILE - 1) - p.center) * p.scale;
-    const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TILE - 1) - p.center) * p.scale;
+    {
+        const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
+        const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
+        for (int a = tid; a < p.P; a += NT) {
+            const AngleWin w = angle_window(cosT[a], sinT[a], xa, xb, ya, yb, p.cI);
+            const int row = ((a / AE) & 1) * AE * W + (a % AE) * W;
+            info[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * 8));
+            kinfo[a] = make_int2(w.kmin, w.kabs);
+        }
+    }
+    __syncthreads();
 
-    for (int p0 = 0; p0 < p.P; p0 += A) {
-        const int na = min(A, p.P - p0);
-        if (tid < na) {
-            const float c = cosT[p0 + tid], s = sinT[p0 + tid];
-            const float t0 = bp_t(xa, ya, c, s), t1 = bp_t(xb, ya, c, s);
-            const float t2 = bp_t(xa, yb, c, s), t3 = bp_t(xb, yb, c, s);
-            const float tmin = fminf(fminf(t0, t1), fminf(t2, t3));
-            const int kmin = (int)floorf(tmin) + p.cI;
-            const int base = tid * p.W + p.cI - kmin;    // entry = floor(t) + base
-            info[tid] = make_float4(c, s, __int_as_float(base * 8), __int_as_float(kmin));
-        }
-        __syncthreads();
-        const long row0 = (long)p0 * p.D;
-        for (int e = tid; e < na * p.W; e += NT) {
-            const int a = (int)(((float)e + 0.5f) * p.invW);
-            const int w = e - a * p.W;
-            const int k = __float_as_int(info[a].w) + w;
-            const long off = row0 + (long)a * p.D;
-            float s0 = 0.0f, s1 = 0.0f;
-            if (in_f16) {
-                if (k >= 0 && k < p.D) s0 = half_bits_to_float(S16[off + k]);
-                if (k + 1 >= 0 && k + 1 < p.D) s1 = half_bits_to_float(S16[off + k + 1]);
-            } else {
-                if (k >= 0 && k < p.D) s0 = S[off + k];
-                if (k + 1 >= 0 && k + 1 < p.D) s1 = S[off + k + 1];
+    const int nch = (p.P + AE - 1) / AE;
+    float s0r[E], s1r[E];
+    int kr[E];
+    auto fetch = [&](int ch) {
+        const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int e = tid + q * NT;
+            s0r[q] = 0.0f;
+            s1r[q] = 0.0f;
+            kr[q] = 0;
+            if (e < nae) {
+                const int a = (int)(((float)e + 0.5f) * p.invW);
+                const int w = e - a * W;
+                const int2 ki = kinfo[p0 + a];
+                const int k = ki.y + w;
+                kr[q] = ki.x + w;
+                const long off = (long)(p0 + a) * p.D;
+                if (in_f16) {
+                    if (k >= 0 && k < p.D) s0r[q] = half_bits_to_float(S16[off + k]);
+                    if (k + 1 >= 0 && k + 1 < p.D) s1r[q] = half_bits_to_float(S16[off + k + 1]);
+                } else {
+                    if (k >= 0 && k < p.D) s0r[q] = S[off + k];
+                    if (k + 1 >= 0 && k + 1 < p.D) s1r[q] = S[off + k + 1];
+                }
             }
-            rows[e] = make_float2(s0, s1 - s0);
         }
-        __syncthreads();
+    };
+    auto commit = [&](int ch) {
+        const int nae = min(AE, p.P - ch * AE) * W;
+        float2* buf = rows + (ch & 1) * AE * W;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int e = tid + q * NT;
+            if (e < nae) {
+                const float d = s1r[q] - s0r[q];
+                buf[e] = make_float2(s0r[q] - (float)kr[q] * d, d);
+            }
+        }
+    };
+
+    fetch(0);
+    commit(0);
+    __syncthreads();
+    const char* rb = reinterpret_cast<const char*>(rows);
+    for (int ch = 0; ch < nch; ch++) {
+        const int p0 = ch * AE, na = min(AE, p.P - p0);
+        if (ch + 1 < nch) fetch(ch + 1);
         if (wave_active) {
-            const char* rb = reinterpret_cast<const char*>(rows);
-            // two angles per step: all 2*PPT LDS reads are issued before the
+            // U angles per step: all U*PPT LDS reads are issued before the
             // first accumulate so their latency overlaps
             int a = 0;
-            for (; a + 1 < na; a += 2) {
-                const float4 in0 = info[a], in1 = info[a + 1];
-                const float xc0 = __fmul_rn(xp, in0.x), xc1 = __fmul_rn(xp, in1.x);
-                const int b0 = __float_as_int(in0.z), b1 = __float_as_int(in1.z);
-                float2 v0[PPT], v1[PPT];
-                float f0[PPT], f1[PPT];
+            for (; a + U <= na; a += U) {
+                float4 in[U];
+                float t[U][PPT];
+                float2 v[U][PPT];
 #pragma unroll
-                for (int r = 0; r < PPT; r++) {
-                    const float t0 = __builtin_fmaf(yp[r], in0.y, xc0);
-                    const float t1 = __builtin_fmaf(yp[r], in1.y, xc1);
-                    const float fl0 = floorf(t0), fl1 = floorf(t1);
-                    f0[r] = t0 - fl0;
-                    f1[r] = t1 - fl1;
-                    v0[r] = *reinterpret_cast<const float2*>(rb + b0 + (int)fl0 * 8);
-                    v1[r] = *reinterpret_cast<const float2*>(rb + b1 + (int)fl1 * 8);
+                for (int u = 0; u < U; u++) in[u] = info[p0 + a + u];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
+                    const int b = __float_as_int(in[u].w);
+#pragma unroll
+                    for (int r = 0; r < PPT; r++) {
+                        t[u][r] = __builtin_fmaf(yp[r], in[u].y, xc);
+                        v[u][r] = *reinterpret_cast<const float2*>(rb + b + floor_to_int(t[u][r]) * 8);
+                    }
                 }
 #pragma unroll
-                for (int r = 0; r < PPT; r++) {
-                    acc[r] += v0[r].x;
-                    acc[r] = __builtin_fmaf(f0[r], v0[r].y, acc[r]);
-                    acc[r] += v1[r].x;
-                    acc[r] = __builtin_fmaf(f1[r], v1[r].y, acc[r]);
-                }
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int r = 0; r < PPT; r++) {
+                        acc[r] += v[u][r].x;
+                        acc[r] = __builtin_fmaf(t[u][r], v[u][r].y, acc[r]);
+                    }
             }
-            if (a < na) {
-                const float4 in = info[a];
-                const float xc = __fmul_rn(xp, in.x);
-                const int base8 = __float_as_int(in.z);
+            for (; a < na; a++) {
+                const float4 in = info[p0 + a];
+                const float xc = __builtin_fmaf(xp, in.x, in.z);
+                const int b = __float_as_int(in.w);
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
                     const float t = __builtin_fmaf(yp[r], in.y, xc);
-                    const float fl = floorf(t);
-                    const float f = t - fl;
-                    const float2 v = *reinterpret_cast<const float2*>(rb + base8 + (int)fl * 8);
+                    const float2 v = *reinterpret_cast<const float2*>(rb + b + floor_to_int(t) * 8);
                     acc[r] += v.x;
-                    acc[r] = __builtin_fmaf(f, v.y, acc[r]);
+                    acc[r] = __builtin_fmaf(t, v.y, acc[r]);
                 }
             }
         }
+        if (ch + 1 < nch) commit(ch + 1);
         __syncthreads();
     }
 #pragma unroll
@@ -179,18 +268,17 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     }
 }
 
-template <int TILE, int PPT, int A>
-__global__ void __launch_bounds__(TILE* TILE / PPT) bp_linear_kernel(const float* __restrict__ sino,
-                                                                     float* __restrict__ img,
-                                                                     const float* __restrict__ cosT,
-                                                                     const float* __restrict__ sinT, BpParams p)
+template <int TX, int TY, int PPT, int E, int U>
+__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __restrict__ sino,
+                                                                 float* __restrict__ img,
+                                                                 const float* __restrict__ cosT,
+                                                                 const float* __restrict__ sinT, BpParams p)
 {
-    extern __shared__ float2 rows[];   // [A][W]
-    __shared__ float4 info[A];         // cos, sin, byte base, window start
-    const int x0 = (blockIdx.x % p.tilesX) * TILE, y0 = (blockIdx.x / p.tilesX) * TILE;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
     const long slice = blockIdx.y;
-    bp_tile<TILE, PPT, A>(sino + slice * (long)p.P * p.D, img + slice * (long)p.N * p.N, cosT, sinT, p, x0, y0, 0,
-                          rows, info);
+    bp_tile<TX, TY, PPT, E, U>(sino + slice * (long)p.P * p.D, img + slice * (long)p.N * p.N, cosT, sinT, p, x0,
+                               y0, 0, smem);
 }
 
 // A job of the persistent uneven batch (device copy of risa_bp_job + derived
@@ -206,15 +294,14 @@ struct BpJobDev {
 // Persistent backprojection over many geometries: a grid sized to the chip
 // pulls tiles (sorted by cost, largest first) from an atomic counter until
 // the list is exhausted; every workgroup reaches the exit.
-template <int TILE, int PPT, int A>
-__global__ void __launch_bounds__(TILE* TILE / PPT) bp_persistent_kernel(const BpJobDev* __restrict__ jobs,
-                                                                         const int2* __restrict__ tiles, int ntiles,
-                                                                         const float* __restrict__ cosT,
-                                                                         const float* __restrict__ sinT,
-                                                                         int* __restrict__ counter)
+template <int TX, int TY, int PPT, int E, int U>
+__global__ void __launch_bounds__(TX* TY / PPT) bp_persistent_kernel(const BpJobDev* __restrict__ jobs,
+                                                                     const int2* __restrict__ tiles, int ntiles,
+                                                                     const float* __restrict__ cosT,
+                                                                     const float* __restrict__ sinT,
+                                                                     int* __restrict__ counter)
 {
-    extern __shared__ float2 rows[];
-    __shared__ float4 info[A];
+    extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -224,35 +311,45 @@ __global__ void __launch_bounds__(TILE* TILE / PPT) bp_persistent_kernel(const B
         if (t >= ntiles) break;
         const int2 w = tiles[t];
         const BpJobDev& j = jobs[w.x];
-        const int x0 = (w.y % j.p.tilesX) * TILE, y0 = (w.y / j.p.tilesX) * TILE;
-        bp_tile<TILE, PPT, A>(j.sino, j.img, cosT + j.lut_off, sinT + j.lut_off, j.p, x0, y0, j.in_f16, rows, info);
+        const int x0 = (w.y % j.p.tilesX) * TX, y0 = (w.y / j.p.tilesX) * TY;
+        bp_tile<TX, TY, PPT, E, U>(j.sino, j.img, cosT + j.lut_off, sinT + j.lut_off, j.p, x0, y0, j.in_f16, smem);
     }
 }
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
+
 // FP16-sinogram variant over B slices of one geometry: each staged entry
 // holds, per slice, the pair (s[k], s[k+1] - s[k]) as two halves; an update
-// of all B slices is one ds_read of 4B bytes and B v_dot2_f32_f16 with the
-// weight pair (1, f).
+// of all B slices is one LDS read of 4B bytes and B v_dot2_f32_f16 with the
+// weight pair (1, f).  Same tiling, windows and double buffering as bp_tile.
 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
 
-template <int TILE, int PPT, int A, int B>
-__global__ void __launch_bounds__(TILE* TILE / PPT) bp_linear_f16_kernel(const uint16_t* __restrict__ sino,
-                                                                         float* __restrict__ img,
-                                                                         const float* __restrict__ cosT,
-                                                                         const float* __restrict__ sinT, int slices,
-                                                                         BpParams p)
+template <int B>
+struct HRow {
+    h2_t v[B];
+};
+
+template <int TX, int TY, int PPT, int B, int E, int U>
+__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint16_t* __restrict__ sino,
+                                                                     float* __restrict__ img,
+                                                                     const float* __restrict__ cosT,
+                                                                     const float* __restrict__ sinT, int slices,
+                                                                     BpParams p)
 {
-    constexpr int NT = TILE * TILE / PPT;
-    constexpr int RS = TILE / PPT;
-    extern __shared__ h2_t rows16[];   // [A][W][B]
-    __shared__ float4 info[A];
+    constexpr int NT = TX * TY / PPT;
+    constexpr int RS = TY / PPT;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int W = p.W, AE = p.AE;
+    HRow<B>* rows = reinterpret_cast<HRow<B>*>(smem);
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(HRow<B>));
+    int2* kinfo = reinterpret_cast<int2*>(info + p.P);
 
     const int tid = threadIdx.x;
-    const int tx = tid % TILE, ty = tid / TILE;
-    const int x0 = (blockIdx.x % p.tilesX) * TILE, y0 = (blockIdx.x / p.tilesX) * TILE;
+    const int tx = tid % TX, ty = tid / TX;
+    const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
     const int b0 = blockIdx.y * B;
     const int nb = min(B, slices - b0);
+    const long plane = (long)p.P * p.D;
 
     const int x = x0 + tx;
     const float xp = ((float)x - p.center) * p.scale;
@@ -282,63 +379,108 @@ __global__ void __launch_bounds__(TILE* TILE / PPT) bp_linear_f16_kernel(const u
     for (int r = 0; r < PPT; r++)
 #pragma unroll
         for (int b = 0; b < B; b++) acc[r][b] = 0.0f;
-
-    const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TILE - 1) - p.center) * p.scale;
-    const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TILE - 1) - p.center) * p.scale;
-    const long plane = (long)p.P * p.D;
-
-    for (int p0 = 0; p0 < p.P; p0 += A) {
-        const int na = min(A, p.P - p0);
-        if (tid < na) {
-            const float c = cosT[p0 + tid], s = sinT[p0 + tid];
-            const float t0 = bp_t(xa, ya, c, s), t1 = bp_t(xb, ya, c, s);
-            const float t2 = bp_t(xa, yb, c, s), t3 = bp_t(xb, yb, c, s);
-            const float tmin = fminf(fminf(t0, t1), fminf(t2, t3));
-            const int kmin = (int)floorf(tmin) + p.cI;
-            const int base = tid * p.W + p.cI - kmin;
-            info[tid] = make_float4(c, s, __int_as_float(base * 4 * B), __int_as_float(kmin));
+    {
+        const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
+        const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
+        for (int a = tid; a < p.P; a += NT) {
+            const AngleWin w = angle_window(cosT[a], sinT[a], xa, xb, ya, yb, p.cI);
+            const int row = ((a / AE) & 1) * AE * W + (a % AE) * W;
+            info[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow<B>)));
+            kinfo[a] = make_int2(w.kmin, w.kabs);
         }
-        __syncthreads();
-        for (int e = tid; e < na * p.W; e += NT) {
-            const int a = (int)(((float)e + 0.5f) * p.invW);
-            const int w = e - a * p.W;
-            const int k = __float_as_int(info[a].w) + w;
-            const long off = (long)(p0 + a) * p.D;
+    }
+    __syncthreads();
+
+    const int nch = (p.P + AE - 1) / AE;
+    HRow<B> pre[E];
+    auto fetch = [&](int ch) {
+        const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
 #pragma unroll
-            for (int b = 0; b < B; b++) {
-                float s0 = 0.0f, s1 = 0.0f;
-                if (b < nb) {
-                    const uint16_t* row = sino + (long)(b0 + b) * plane + off;
-                    if (k >= 0 && k < p.D) s0 = half_bits_to_float(row[k]);
-                    if (k + 1 >= 0 && k + 1 < p.D) s1 = half_bits_to_float(row[k + 1]);
+        for (int q = 0; q < E; q++) {
+            const int e = tid + q * NT;
+            if (e < nae) {
+                const int a = (int)(((float)e + 0.5f) * p.invW);
+                const int k = kinfo[p0 + a].y + e - a * W;
+                const long off = (long)(p0 + a) * p.D;
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    float s0 = 0.0f, s1 = 0.0f;
+                    if (b < nb) {
+                        const uint16_t* row = sino + (long)(b0 + b) * plane + off;
+                        if (k >= 0 && k < p.D) s0 = half_bits_to_float(row[k]);
+                        if (k + 1 >= 0 && k + 1 < p.D) s1 = half_bits_to_float(row[k + 1]);
+                    }
+                    pre[q].v[b].x = (_Float16)s0;
+                    pre[q].v[b].y = (_Float16)(s1 - s0);
                 }
-                h2_t v;
-                v.x = (_Float16)s0;
-                v.y = (_Float16)(s1 - s0);
-                rows16[(long)e * B + b] = v;
             }
         }
-        __syncthreads();
+    };
+    auto commit = [&](int ch) {
+        const int nae = min(AE, p.P - ch * AE) * W;
+        HRow<B>* buf = rows + (ch & 1) * AE * W;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int e = tid + q * NT;
+            if (e < nae) buf[e] = pre[q];
+        }
+    };
+    fetch(0);
+    commit(0);
+    __syncthreads();
+    const char* rb = reinterpret_cast<const char*>(rows);
+    for (int ch = 0; ch < nch; ch++) {
+        const int p0 = ch * AE, na = min(AE, p.P - p0);
+        if (ch + 1 < nch) fetch(ch + 1);
         if (wave_active) {
-            const char* rb = reinterpret_cast<const char*>(rows16);
-#pragma unroll 2
-            for (int a = 0; a < na; a++) {
-                const float4 in = info[a];
-                const float xc = __fmul_rn(xp, in.x);
-                const int base = __float_as_int(in.z);
+            int a = 0;
+            for (; a + U <= na; a += U) {
+                float4 in[U];
+                HRow<B> v[U][PPT];
+                h2_t wgt[U][PPT];
+#pragma unroll
+                for (int u = 0; u < U; u++) in[u] = info[p0 + a + u];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
+                    const char* row = rb + __float_as_int(in[u].w);
+#pragma unroll
+                    for (int r = 0; r < PPT; r++) {
+                        int k;
+                        float f;
+                        bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
+                        wgt[u][r].x = (_Float16)1.0f;
+                        wgt[u][r].y = (_Float16)f;
+                        v[u][r] = *reinterpret_cast<const HRow<B>*>(row + k * (int)sizeof(HRow<B>));
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int r = 0; r < PPT; r++)
+#pragma unroll
+                        for (int b = 0; b < B; b++)
+                            acc[r][b] = __builtin_amdgcn_fdot2(wgt[u][r], v[u][r].v[b], acc[r][b], false);
+            }
+            for (; a < na; a++) {
+                const float4 in = info[p0 + a];
+                const float xc = __builtin_fmaf(xp, in.x, in.z);
+                const char* row = rb + __float_as_int(in.w);
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
-                    const float t = __builtin_fmaf(yp[r], in.y, xc);
-                    const float fl = floorf(t);
+                    int k;
+                    float f;
+                    bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
                     h2_t wgt;
                     wgt.x = (_Float16)1.0f;
-                    wgt.y = (_Float16)(t - fl);
-                    const h2_t* v = reinterpret_cast<const h2_t*>(__builtin_assume_aligned(rb + base + (int)fl * (4 * B), 4 * B));
+                    wgt.y = (_Float16)f;
+                    const HRow<B> v = *reinterpret_cast<const HRow<B>*>(row + k * (int)sizeof(HRow<B>));
 #pragma unroll
-                    for (int b = 0; b < B; b++) acc[r][b] = __builtin_amdgcn_fdot2(wgt, v[b], acc[r][b], false);
+                    for (int b = 0; b < B; b++) acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.v[b], acc[r][b], false);
                 }
             }
         }
+        if (ch + 1 < nch) commit(ch + 1);
         __syncthreads();
     }
     for (int b = 0; b < nb; b++)
@@ -350,7 +492,6 @@ __global__ void __launch_bounds__(TILE* TILE / PPT) bp_linear_f16_kernel(const u
                     use[r] ? acc[r][b] * p.norm * p.out_scale : p.maskingValue;
         }
 }
-
 // Nearest-neighbour variants (backProjectNearest :305-336, backProjectTex
 // :273-302), one thread per pixel.
 __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restrict__ img,
@@ -406,10 +547,10 @@ __global__ void bp_adjoint_kernel(const float* __restrict__ img, float* __restri
             const int x = alongX ? w : u, y = alongX ? u : w;
             if (p.mask && bp_outside(x, y, p.N, p.center)) continue;
             const float xp = ((float)x - p.center) * p.scale, yp = ((float)y - p.center) * p.scale;
-            const float t = bp_t(xp, yp, c, s);
-            const float fl = floorf(t);
-            const int kk = (int)fl + p.cI;
-            const float f = t - fl;
+            int kk;
+            float f;
+            bp_split(bp_t(xp, yp, c, s), kk, f);
+            kk += p.cI;
             const float v = img[(long)y * p.N + x];
             if (kk == k) acc += (1.0f - f) * v;
             else if (kk + 1 == k) acc += f * v;
@@ -466,38 +607,59 @@ __global__ void mask_apply_kernel(float* __restrict__ img, int N, float center, 
 // ----------------------------------------------------------------------------
 namespace risa {
 
+// A variant: tile edge, pixels per thread, staged entries per thread (E)
+// and the cap on angles per chunk.  The chunk holds AE = min(amax,
+// E * threads / W) angles so one thread stages at most E entries.
 struct BpVariant {
-    int tile, ppt, a;
+    int tx, ty, ppt, e, u, amax;
     void (*fn)(const float*, float*, const float*, const float*, BpParams);
 };
 
+#define RISA_BPV(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_linear_kernel<TX, TY, PPT, E, U>}
 static const BpVariant kVariants[] = {
-    {32, 4, 32, bp_linear_kernel<32, 4, 32>}, {32, 2, 32, bp_linear_kernel<32, 2, 32>},
-    {16, 1, 32, bp_linear_kernel<16, 1, 32>}, {16, 2, 64, bp_linear_kernel<16, 2, 64>},
-    {32, 4, 64, bp_linear_kernel<32, 4, 64>}, {16, 4, 64, bp_linear_kernel<16, 4, 64>},
-    {16, 2, 32, bp_linear_kernel<16, 2, 32>}, {32, 8, 32, bp_linear_kernel<32, 8, 32>},
+    RISA_BPV(32, 32, 4, 8, 4, 64), RISA_BPV(32, 16, 2, 8, 4, 64), RISA_BPV(16, 16, 1, 4, 4, 64),
+    RISA_BPV(32, 32, 8, 16, 2, 64), RISA_BPV(64, 16, 4, 16, 4, 64), RISA_BPV(32, 16, 4, 16, 4, 64),
+    RISA_BPV(16, 32, 2, 8, 4, 64), RISA_BPV(32, 32, 4, 8, 2, 64), RISA_BPV(16, 16, 2, 8, 4, 64),
+    RISA_BPV(32, 8, 1, 4, 4, 64),
 };
+#undef RISA_BPV
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 struct BpF16Variant {
-    int tile, ppt, a, b;
+    int tx, ty, ppt, b, e, u, amax;
     void (*fn)(const uint16_t*, float*, const float*, const float*, int, BpParams);
 };
+#define RISA_BPH(TX, TY, PPT, B, E, U, AMAX) \
+    {TX, TY, PPT, B, E, U, AMAX, bp_linear_f16_kernel<TX, TY, PPT, B, E, U>}
 static const BpF16Variant kF16Variants[] = {
-    {32, 4, 32, 4, bp_linear_f16_kernel<32, 4, 32, 4>},
-    {16, 2, 32, 4, bp_linear_f16_kernel<16, 2, 32, 4>},
-    {32, 4, 32, 2, bp_linear_f16_kernel<32, 4, 32, 2>},
-    {16, 1, 32, 4, bp_linear_f16_kernel<16, 1, 32, 4>},
-    {32, 2, 32, 4, bp_linear_f16_kernel<32, 2, 32, 4>},
-    {32, 4, 16, 8, bp_linear_f16_kernel<32, 4, 16, 8>},
+    RISA_BPH(32, 16, 2, 4, 4, 2, 32), RISA_BPH(16, 16, 1, 4, 2, 4, 32), RISA_BPH(32, 32, 4, 2, 4, 2, 64),
+    RISA_BPH(32, 16, 2, 2, 4, 4, 64), RISA_BPH(32, 16, 4, 4, 4, 2, 32), RISA_BPH(16, 16, 2, 4, 4, 2, 32),
+    RISA_BPH(32, 8, 1, 4, 2, 4, 32),  RISA_BPH(16, 16, 1, 8, 2, 2, 16),
 };
+#undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
 
-// Largest window any tile of edge `tile` needs for detector scale `scale`:
-// floor(t_max) - floor(t_min) + 1 <= ceil((tile-1) * scale * (|c|+|s|)) + 2.
-static int window_entries(int tile, float scale)
+// Largest window a tx x ty tile needs for detector scale `scale`: the span of
+// t over the tile is at most scale * ((tx-1)|c| + (ty-1)|s|)
+// <= scale * sqrt((tx-1)^2 + (ty-1)^2), and floor(t_max) - floor(t_min) + 1
+// <= ceil(span) + 2 (plus slack for rounding of t and of the table).
+static int window_entries(int tx, int ty, float scale)
 {
-    return (int)std::ceil((tile - 1) * (double)scale * 1.41422 + 1e-3) + 2;
+    const double diag = std::sqrt((double)(tx - 1) * (tx - 1) + (double)(ty - 1) * (ty - 1));
+    return (int)std::ceil(diag * (double)scale * 1.00001 + 1e-2) + 2;
+}
+
+// Angles per chunk and the dynamic LDS bytes of a tile:
+// info[P] (16 B) + 2 buffers x AE x W entries of entry_bytes.
+constexpr size_t kMaxLds = 64 * 1024;
+static size_t tile_lds(int P, int AE, int W, int entry_bytes) { return (size_t)P * 24 + (size_t)2 * AE * W * entry_bytes; }
+// AE also shrinks so the tile fits the LDS budget (when P leaves room at all).
+static int chunk_angles(int threads, int e, int amax, int W, int P = 0, int entry_bytes = 8)
+{
+    int ae = std::min(amax, e * threads / W);
+    const long room = (long)kMaxLds - (long)P * 24;
+    if (room > 0) ae = std::min<long>(ae, room / (2L * W * entry_bytes));
+    return std::max(1, ae);
 }
 
 }  // namespace risa
@@ -514,7 +676,7 @@ struct risa_bp_s {
 
 using namespace risa;
 
-static BpParams make_params(const risa_bp_s* h, int tile)
+static BpParams make_params(const risa_bp_s* h, int tx, int ty)
 {
     BpParams p{};
     p.N = h->g.numberOfPixels;
@@ -526,8 +688,8 @@ static BpParams make_params(const risa_bp_s* h, int tile)
     p.center = h->lut.center;
     p.mask = h->fuse_mask ? 1 : 0;
     p.maskingValue = (float)h->g.maskingValue;
-    p.W = window_entries(tile, p.scale);
-    p.tilesX = ceil_div(p.N, tile);
+    p.W = window_entries(tx, ty, p.scale);
+    p.tilesX = ceil_div(p.N, tx);
     p.invW = 1.0f / (float)p.W;
     p.out_scale = 1.0f;
     return p;
@@ -560,11 +722,13 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
 static void launch_linear(risa_bp_t h, int variant, const float* sino, float* img, int slices, hipStream_t s)
 {
     const BpVariant& v = kVariants[variant];
-    BpParams p = make_params(h, v.tile);
-    const size_t lds = (size_t)v.a * p.W * sizeof(float2);
-    if (lds > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "backprojection window too large for LDS");
-    dim3 grid(p.tilesX * p.tilesX, slices);
-    hipLaunchKernelGGL(v.fn, grid, dim3(v.tile * v.tile / v.ppt), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, p);
+    BpParams p = make_params(h, v.tx, v.ty);
+    const int threads = v.tx * v.ty / v.ppt;
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 8);
+    const size_t lds = tile_lds(p.P, p.AE, p.W, sizeof(float2));
+    if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
+    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), slices);
+    hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, p);
     check_launch("bp_linear_kernel");
 }
 
@@ -574,7 +738,7 @@ extern "C" int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void*
         if (!h || !sino || !img) fail(RISA_ERR_ARG, "null argument");
         hipStream_t s = as_stream(stream);
         if (h->g.useTextureMemory || h->g.interpolationType == 1) {
-            BpParams p = make_params(h, 16);
+            BpParams p = make_params(h, 16, 16);
             dim3 grid(ceil_div(p.N, 16), ceil_div(p.N, 16));
             hipLaunchKernelGGL(bp_nearest_kernel, grid, dim3(16, 16), 0, s, sino, img, h->cosT.ptr, h->sinT.ptr, p,
                                h->g.useTextureMemory ? 1 : 0);
@@ -595,13 +759,14 @@ static void launch_f16(risa_bp_t h, int variant, const uint16_t* sino, float in_
                        hipStream_t s)
 {
     const BpF16Variant& v = kF16Variants[variant];
-    BpParams p = make_params(h, v.tile);
+    BpParams p = make_params(h, v.tx, v.ty);
     p.out_scale = 1.0f / in_scale;
-    const size_t lds = (size_t)v.a * p.W * v.b * 4;
-    if (lds > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "backprojection window too large for LDS");
-    dim3 grid(p.tilesX * p.tilesX, ceil_div(slices, v.b));
-    hipLaunchKernelGGL(v.fn, grid, dim3(v.tile * v.tile / v.ppt), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices,
-                       p);
+    const int threads = v.tx * v.ty / v.ppt;
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 4 * v.b);
+    const size_t lds = tile_lds(p.P, p.AE, p.W, 4 * v.b);
+    if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
+    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.b));
+    hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_f16_kernel");
 }
 
@@ -618,7 +783,7 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
 {
     return guarded("risa_hip_bp_adjoint_run", [&] {
         if (!h || !img || !sino) fail(RISA_ERR_ARG, "null argument");
-        BpParams p = make_params(h, 16);
+        BpParams p = make_params(h, 16, 16);
         dim3 grid(ceil_div(p.D, 64), p.P);
         hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64), 0, as_stream(stream), img, sino, h->cosT.ptr,
                            h->sinT.ptr, p);
@@ -646,17 +811,16 @@ extern "C" int risa_hip_bp_get_variant(risa_bp_t h, int* variant)
     });
 }
 
-// Autotuner: times every variant on this geometry (FP32 single slice and the
-// FP16 batched form) with HIP events and keeps the fastest.
-extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
+// `slices` FP32 slices per timed launch stand for that many frames in flight
+// (the chain keeps several frames of one geometry in flight at once).
+int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices)
 {
     return guarded("risa_hip_bp_autotune", [&] {
-        if (!h) fail(RISA_ERR_ARG, "null handle");
+        if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
         DeviceGuard dg(h->device);
-        hipStream_t s = as_stream(stream);
         const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
-        const int B = 8;
-        DeviceBuffer<float> sino((size_t)P * D);
+        const int B = std::max(8, slices);
+        DeviceBuffer<float> sino((size_t)P * D * slices);
         DeviceBuffer<uint16_t> sino16((size_t)P * D * B);
         DeviceBuffer<float> img((size_t)N * N * B);
         RISA_HIP_CHECK(hipMemsetAsync(sino.ptr, 0, sino.count * sizeof(float), s));
@@ -676,21 +840,36 @@ extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
         };
         float best = 1e30f;
         for (int v = 0; v < kNumVariants; v++) {
-            BpParams p = make_params(h, kVariants[v].tile);
-            if ((size_t)kVariants[v].a * p.W * sizeof(float2) > 64 * 1024) continue;
-            const float ms = time_it([&] { launch_linear(h, v, sino.ptr, img.ptr, 1, s); });
+            float ms;
+            try {
+                ms = time_it([&] { launch_linear(h, v, sino.ptr, img.ptr, slices, s); });
+            } catch (const Error& e) {
+                if (e.code != RISA_ERR_UNSUPPORTED) throw;
+                continue;   // this tile shape does not fit the geometry
+            }
             if (ms < best) { best = ms; h->variant = v; }
         }
         best = 1e30f;
         for (int v = 0; v < kNumF16Variants; v++) {
-            BpParams p = make_params(h, kF16Variants[v].tile);
-            if ((size_t)kF16Variants[v].a * p.W * kF16Variants[v].b * 4 > 64 * 1024) continue;
-            const float ms = time_it([&] { launch_f16(h, v, sino16.ptr, 1.0f, B, img.ptr, s); });
+            float ms;
+            try {
+                ms = time_it([&] { launch_f16(h, v, sino16.ptr, 1.0f, B, img.ptr, s); });
+            } catch (const Error& e) {
+                if (e.code != RISA_ERR_UNSUPPORTED) throw;
+                continue;
+            }
             if (ms < best) { best = ms; h->variant_f16 = v; }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     });
+}
+
+// Autotuner: times every tile shape on this geometry (FP32 single slice and
+// the FP16 batched form) with HIP events and keeps the fastest.
+extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
+{
+    return risa_bp_autotune_slices(h, as_stream(stream), 1);
 }
 
 extern "C" int risa_hip_bp_destroy(risa_bp_t h)
@@ -754,15 +933,15 @@ extern "C" int risa_hip_mask_destroy(risa_mask_t h)
 namespace risa {
 
 struct PersistentVariant {
-    int tile, ppt, a;
+    int tx, ty, ppt, e, u, amax;
     void (*fn)(const BpJobDev*, const int2*, int, const float*, const float*, int*);
 };
+#define RISA_BPP(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_persistent_kernel<TX, TY, PPT, E, U>}
 static const PersistentVariant kPersistent[] = {
-    {32, 4, 32, bp_persistent_kernel<32, 4, 32>},
-    {16, 2, 32, bp_persistent_kernel<16, 2, 32>},
-    {32, 2, 32, bp_persistent_kernel<32, 2, 32>},
-    {16, 1, 32, bp_persistent_kernel<16, 1, 32>},
+    RISA_BPP(32, 32, 4, 8, 4, 64), RISA_BPP(32, 16, 2, 8, 4, 64), RISA_BPP(16, 16, 1, 4, 4, 64),
+    RISA_BPP(32, 16, 4, 16, 4, 64),
 };
+#undef RISA_BPP
 static constexpr int kNumPersistent = sizeof(kPersistent) / sizeof(kPersistent[0]);
 
 }  // namespace risa
@@ -787,7 +966,8 @@ static void batched_layout(risa_bp_batched_s* h)
     const PersistentVariant& v = kPersistent[h->variant];
     std::vector<BpJobDev> dj(h->jobs.size());
     std::vector<std::pair<long, int2>> order;
-    int maxW = 0;
+    size_t lds = 0;
+    const int threads = v.tx * v.ty / v.ppt;
     for (size_t i = 0; i < h->jobs.size(); i++) {
         const risa_bp_job& j = h->jobs[i];
         BpParams p{};
@@ -798,23 +978,25 @@ static void batched_layout(risa_bp_batched_s* h)
         p.center = (float)((j.N - 1.0) * 0.5);
         p.mask = j.mask ? 1 : 0;
         p.maskingValue = j.maskingValue;
-        p.W = window_entries(v.tile, p.scale);
-        p.tilesX = ceil_div(j.N, v.tile);
+        p.W = window_entries(v.tx, v.ty, p.scale);
+        p.tilesX = ceil_div(j.N, v.tx);
+        const int tilesY = ceil_div(j.N, v.ty);
         p.invW = 1.0f / (float)p.W;
         p.out_scale = j.dtype == RISA_F16 ? 1.0f / j.in_scale : 1.0f;
-        maxW = std::max(maxW, p.W);
+        p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 8);
+        lds = std::max(lds, tile_lds(p.P, p.AE, p.W, sizeof(float2)));
         dj[i].sino = j.sino;
         dj[i].img = j.img;
         dj[i].lut_off = h->lut_off[i];
         dj[i].in_f16 = j.dtype == RISA_F16 ? 1 : 0;
         dj[i].p = p;
         const float c = p.center;
-        for (int t = 0; t < p.tilesX * p.tilesX; t++) {
-            const int x0 = (t % p.tilesX) * v.tile, y0 = (t / p.tilesX) * v.tile;
+        for (int t = 0; t < p.tilesX * tilesY; t++) {
+            const int x0 = (t % p.tilesX) * v.tx, y0 = (t / p.tilesX) * v.ty;
             // pixels of the tile inside the image (and the circle when masked)
             long px = 0;
-            for (int y = y0; y < std::min(j.N, y0 + v.tile); y += 4)
-                for (int x = x0; x < std::min(j.N, x0 + v.tile); x += 4) {
+            for (int y = y0; y < std::min(j.N, y0 + v.ty); y += 4)
+                for (int x = x0; x < std::min(j.N, x0 + v.tx); x += 4) {
                     const float dX = (float)x - c, dY = (float)y - c;
                     if (!p.mask || (double)(dX * dX + dY * dY) <= (double)j.N * j.N * 0.25) px++;
                 }
@@ -825,8 +1007,8 @@ static void batched_layout(risa_bp_batched_s* h)
     std::vector<int2> tl(order.size());
     for (size_t i = 0; i < order.size(); i++) tl[i] = order[i].second;
     h->ntiles = (int)tl.size();
-    h->lds = (size_t)v.a * maxW * sizeof(float2);
-    if (h->lds > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "backprojection window too large for LDS");
+    h->lds = lds;
+    if (h->lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tiles need %zu B of LDS", lds);
     h->djobs.resize(dj.size());
     h->djobs.upload(dj);
     h->tiles.resize(tl.size());
@@ -834,7 +1016,7 @@ static void batched_layout(risa_bp_batched_s* h)
     int cus = 0, per_cu = 0;
     RISA_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     RISA_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(v.fn),
-                                                                 v.tile * v.tile / v.ppt, h->lds));
+                                                                 v.tx * v.ty / v.ppt, h->lds));
     h->grid = std::max(1, std::min(h->ntiles, cus * std::max(1, per_cu)));
 }
 
@@ -898,7 +1080,7 @@ static void launch_persistent(risa_bp_batched_t h, hipStream_t s)
     const PersistentVariant& v = kPersistent[h->variant];
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, h->counter.ptr);
     check_launch("reset_counter_kernel");
-    hipLaunchKernelGGL(v.fn, dim3(h->grid), dim3(v.tile * v.tile / v.ppt), h->lds, s, h->djobs.ptr, h->tiles.ptr,
+    hipLaunchKernelGGL(v.fn, dim3(h->grid), dim3(v.tx * v.ty / v.ppt), h->lds, s, h->djobs.ptr, h->tiles.ptr,
                        h->ntiles, h->cosT.ptr, h->sinT.ptr, h->counter.ptr);
     check_launch("bp_persistent_kernel");
 }

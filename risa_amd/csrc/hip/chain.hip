@@ -6,6 +6,10 @@
 // stream-ordered, and a captured hipGraph replays the launch sequence.
 //
 // PLAIN: frame by frame, FP32 throughout (FFT filter, FP32 backprojector).
+//        Each frame is its own launch sequence; frames are dealt round-robin
+//        to `lanes` streams (own scratch buffers each) forked from and joined
+//        back into the caller's stream, so several frames are in flight --
+//        the way the reference overlaps frames across its per-stage threads.
 // MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA with FP16
 //        output, FP16-sinogram / FP32-accumulate backprojection.
 #include "device_common.h"
@@ -15,27 +19,33 @@
 #include <vector>
 
 int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows, hipStream_t s);
-int risa_bp_run_batch_f32(risa_bp_t h, const float* sino, float* img, int slices, hipStream_t s);
+int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices);
 
 namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
-}
+constexpr int kPlainLanes = 4;            // frames in flight in PLAIN mode
+}  // namespace
 
 struct risa_chain_s {
     int device = 0;
     int mode = 0;
     int max_frames = 0;
+    int lanes = 1;
     risa_geom g{};
     risa_attenuation_t att = nullptr;
     risa_fan2para_t f2p = nullptr;
     risa_filter_t filt = nullptr;
     risa_bp_t bp = nullptr;
-    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;
+    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;   // PLAIN: one slot per lane
     risa::DeviceBuffer<uint16_t> f16_buf;
+    std::vector<hipStream_t> lane_streams;
+    std::vector<hipEvent_t> lane_events;   // [0] fork, [1..lanes] joins
     std::map<std::tuple<const void*, const void*, int, std::vector<int32_t>>, hipGraphExec_t> graphs;
     ~risa_chain_s()
     {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (hipEvent_t e : lane_events) (void)hipEventDestroy(e);
+        for (hipStream_t s : lane_streams) (void)hipStreamDestroy(s);
         if (bp) risa_hip_bp_destroy(bp);
         if (filt) risa_hip_filter_destroy(filt);
         if (f2p) risa_hip_fan2para_destroy(f2p);
@@ -70,12 +80,28 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
             ok(risa_hip_bp_create(g, device, &h->bp));
             const size_t fan = (size_t)g->numberOfFanDetectors * g->numberOfFanProjections;
             const size_t par = (size_t)g->numberOfParallelDetectors * g->numberOfParallelProjections;
-            const int slots = mode == RISA_CHAIN_PLAIN ? 1 : max_frames;
-            h->att_buf.resize(fan * slots);
-            h->par_buf.resize(par * slots);
-            if (mode == RISA_CHAIN_MIXED) h->f16_buf.resize(par * slots);
-            else h->filt_buf.resize(par);
-            ok(risa_hip_bp_autotune(h->bp, nullptr));
+            if (mode == RISA_CHAIN_PLAIN) {
+                h->lanes = std::min(kPlainLanes, max_frames);
+                h->att_buf.resize(fan * h->lanes);
+                h->par_buf.resize(par * h->lanes);
+                h->filt_buf.resize(par * h->lanes);
+                for (int k = 0; k < h->lanes; k++) {
+                    hipStream_t s;
+                    RISA_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                    h->lane_streams.push_back(s);
+                }
+                for (int k = 0; k <= h->lanes; k++) {
+                    hipEvent_t e;
+                    RISA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                    h->lane_events.push_back(e);
+                }
+            } else {
+                h->att_buf.resize(fan * max_frames);
+                h->par_buf.resize(par * max_frames);
+                h->f16_buf.resize(par * max_frames);
+            }
+            // tile shape tuned for the concurrency the chain runs the kernel at
+            ok(risa_bp_autotune_slices(h->bp, nullptr, h->lanes));
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
             delete h;
@@ -92,11 +118,23 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
     const size_t par = (size_t)g.numberOfParallelDetectors * g.numberOfParallelProjections;
     const size_t out = (size_t)g.numberOfPixels * g.numberOfPixels;
     if (h->mode == RISA_CHAIN_PLAIN) {
+        const int lanes = std::min(h->lanes, frames);
+        RISA_HIP_CHECK(hipEventRecord(h->lane_events[0], s));
+        for (int k = 0; k < lanes; k++) RISA_HIP_CHECK(hipStreamWaitEvent(h->lane_streams[k], h->lane_events[0], 0));
         for (int f = 0; f < frames; f++) {
-            ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], h->att_buf.ptr, s));
-            ok(risa_hip_fan2para_run(h->f2p, h->att_buf.ptr, planes[f], h->par_buf.ptr, s));
-            ok(risa_filter_run_oop(h->filt, h->par_buf.ptr, h->filt_buf.ptr, g.numberOfParallelProjections, s));
-            ok(risa_hip_bp_run(h->bp, h->filt_buf.ptr, img + f * out, s));
+            const int k = f % lanes;
+            hipStream_t ls = h->lane_streams[k];
+            float* att = h->att_buf.ptr + k * fan;
+            float* pb = h->par_buf.ptr + k * par;
+            float* fb = h->filt_buf.ptr + k * par;
+            ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], att, ls));
+            ok(risa_hip_fan2para_run(h->f2p, att, planes[f], pb, ls));
+            ok(risa_filter_run_oop(h->filt, pb, fb, g.numberOfParallelProjections, ls));
+            ok(risa_hip_bp_run(h->bp, fb, img + f * out, ls));
+        }
+        for (int k = 0; k < lanes; k++) {
+            RISA_HIP_CHECK(hipEventRecord(h->lane_events[k + 1], h->lane_streams[k]));
+            RISA_HIP_CHECK(hipStreamWaitEvent(s, h->lane_events[k + 1], 0));
         }
         return;
     }
@@ -104,7 +142,6 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
     ok(risa_hip_fan2para_run_batch(h->f2p, h->att_buf.ptr, planes, frames, h->par_buf.ptr, s));
     ok(risa_hip_filter_run_f16(h->filt, h->par_buf.ptr, h->f16_buf.ptr, frames, kF16SinoScale, s));
     ok(risa_hip_bp_run_f16(h->bp, h->f16_buf.ptr, kF16SinoScale, frames, img, s));
-    (void)par;
 }
 
 extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img,
@@ -114,6 +151,8 @@ extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int
         if (!h || !raw || !planes || !img || frames < 0) fail(RISA_ERR_ARG, "bad argument");
         if (frames > h->max_frames) fail(RISA_ERR_ARG, "frames (%d) exceeds max_frames (%d)", frames, h->max_frames);
         if (frames == 0) return;
+        for (int f = 0; f < frames; f++)
+            if (planes[f] < 0 || planes[f] >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
         hipStream_t s = as_stream(stream);
         if (!use_graph) {
             enqueue(h, raw, planes, frames, img, s);

@@ -134,9 +134,13 @@ def test_backprojection_variants_match_oracle(risa, orc, geom_c0, cuda):
     m = disk_mask(g.numberOfPixels)
     bp = device.Backprojection(g)
     st = _t(filt, cuda)
-    for v in range(8):
+    for v in range(64):
+        try:
+            bp.variant = v
+        except risa.RisaError:
+            assert v >= 4
+            break
         print(f"case: backprojection C0 variant {v}", flush=True)
-        bp.variant = v
         img = torch.full(g.img_shape, float("nan"), dtype=torch.float32, device=cuda)
         bp(st, img)
         _sync()
