@@ -65,6 +65,48 @@ def rel_rmse(a, ref, m):
     return float(np.sqrt(np.mean((a - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
 
 
+def kernel_only(args, g, dev):
+    """One kernel launched back to back on one stream, timed with HIP events on
+    that stream: the figure a rocprofv3 kernel trace of the same command must
+    agree with.  `copy` is a plain device copy of known size, the scale for
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    from risa_amd import device
+
+    N, P, D = g.numberOfPixels, g.numberOfParallelProjections, g.numberOfParallelDetectors
+    stream = torch.cuda.Stream()
+    rng = np.random.default_rng(1)
+    with torch.cuda.stream(stream):
+        if args.kernel in ("bp", "bp16"):
+            bp = device.Backprojection(g)
+            bp.autotune()
+            if args.kernel == "bp":
+                sino = torch.from_numpy((rng.standard_normal((P, D)) * 0.01).astype(np.float32)).to(dev)
+                img = torch.empty((N, N), dtype=torch.float32, device=dev)
+                fn, per, what = (lambda: bp(sino, img)), 1, f"bp FP32, 1 slice {N}^2 x {P} angles, variant {bp.variant}"
+            else:
+                B = 16
+                sino = torch.from_numpy((rng.standard_normal((B, P, D)) * 2.0).astype(np.float16)).to(dev)
+                img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
+                fn, per, what = (lambda: bp.f16(sino, 256.0, B, img)), B, f"bp FP16 batched, {B} slices {N}^2 x {P}"
+        else:
+            n = 64 << 20
+            src = torch.ones(n, dtype=torch.float32, device=dev)
+            dst = torch.empty_like(src)
+            fn, per, what = (lambda: dst.copy_(src)), 1, f"copy {4 * n} bytes read + {4 * n} written"
+        for _ in range(5):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.launches):
+            fn()
+        e1.record(stream)
+    stream.synchronize()
+    us = e0.elapsed_time(e1) / args.launches * 1e3
+    print(json.dumps({"kernel": args.kernel, "what": what, "launches": args.launches,
+                      "us_per_launch": round(us, 2), "us_per_slice": round(us / per, 2)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +118,9 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=16, help="frame-planes in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
+                    help="time one kernel alone (for rocprofv3 runs) and exit")
+    ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -93,6 +138,10 @@ def main():
     planes = g.numberOfPlanes
     N = g.numberOfPixels
     fan_n = g.numberOfFanDetectors * g.numberOfFanProjections
+
+    if args.kernel:
+        kernel_only(args, g, dev)
+        return
 
     t0 = time.time()
     dead = synth.dead_columns(g.numberOfFanDetectors, 0.01, 7001)
@@ -229,6 +278,15 @@ def main():
         log(f"cpu baseline: {cpu}")
         oc.close()
 
+    # HBM bytes per launch of the backprojector from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size)
+    traffic, traffic_kernel = None, None
+    tf = ROOT / "profiles" / "bp_c1_hbm_traffic.json"
+    if tf.exists():
+        tj = json.loads(tf.read_text())
+        if "bp" in tj:
+            traffic, traffic_kernel = tj["bp"]["bytes"], tj["bp"]["kernel"]
+
     plain_value = total / (ms_plain / 1e3)
     mixed_value = total / (ms_mixed / 1e3)
     out = {
@@ -255,7 +313,9 @@ def main():
         "roofline": {"bound": "mfma", "resource": "fp32 vector ALU (backprojection is compute-bound; "
                                                   "AI ~850 flop/B >> ridge ~20)",
                      "kernel": "bp_linear_kernel", "achieved": round(bp_tflops, 2), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(bp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(bp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/)",
+                     "traffic_kernel": traffic_kernel,
                      "launch_us": round(bp_ms * 1e3, 2), "flop_per_launch": bp_flops},
         "bp_hbm": {"achieved": round(bp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(bp_gbs / HBM_PEAK_GBS, 5), "bytes_per_launch": bp_bytes},
