@@ -526,8 +526,14 @@ __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restr
 
 // Ray-driven projector: the exact transpose of bp_linear_kernel.  Thread per
 // (angle, detector k); walks the pixels whose floor(t) + cI is k or k - 1.
+// Ray-driven transpose of the linear backprojector.  Each pixel's t is
+// formed exactly as the forward kernel forms it for the tile (TX x TY) the
+// pixel falls in -- relative to that tile's integer origin T -- so the weights
+// are the forward's own and <BP s, x> = <s, BP^T x> holds to accumulation
+// rounding.
 __global__ void bp_adjoint_kernel(const float* __restrict__ img, float* __restrict__ sino,
-                                  const float* __restrict__ cosT, const float* __restrict__ sinT, BpParams p)
+                                  const float* __restrict__ cosT, const float* __restrict__ sinT, BpParams p,
+                                  int TX, int TY)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int a = blockIdx.y;
@@ -547,10 +553,15 @@ __global__ void bp_adjoint_kernel(const float* __restrict__ img, float* __restri
             const int x = alongX ? w : u, y = alongX ? u : w;
             if (p.mask && bp_outside(x, y, p.N, p.center)) continue;
             const float xp = ((float)x - p.center) * p.scale, yp = ((float)y - p.center) * p.scale;
-            int kk;
-            float f;
-            bp_split(bp_t(xp, yp, c, s), kk, f);
-            kk += p.cI;
+            const int x0 = (x / TX) * TX, y0 = (y / TY) * TY;
+            const AngleWin aw = angle_window(c, s, ((float)x0 - p.center) * p.scale,
+                                             ((float)(x0 + TX - 1) - p.center) * p.scale,
+                                             ((float)y0 - p.center) * p.scale,
+                                             ((float)(y0 + TY - 1) - p.center) * p.scale, p.cI);
+            const float tl = __builtin_fmaf(yp, s, __builtin_fmaf(xp, c, aw.negT));
+            int kk = floor_to_int(tl);
+            const float f = tl - (float)kk;
+            kk += (int)(-aw.negT) + p.cI;
             const float v = img[(long)y * p.N + x];
             if (kk == k) acc += (1.0f - f) * v;
             else if (kk + 1 == k) acc += f * v;
@@ -783,10 +794,12 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
 {
     return guarded("risa_hip_bp_adjoint_run", [&] {
         if (!h || !img || !sino) fail(RISA_ERR_ARG, "null argument");
-        BpParams p = make_params(h, 16, 16);
+        // the transpose of the FP32 linear kernel the handle runs: same tile shape
+        const BpVariant& v = kVariants[h->variant];
+        BpParams p = make_params(h, v.tx, v.ty);
         dim3 grid(ceil_div(p.D, 64), p.P);
         hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64), 0, as_stream(stream), img, sino, h->cosT.ptr,
-                           h->sinT.ptr, p);
+                           h->sinT.ptr, p, v.tx, v.ty);
         check_launch("bp_adjoint_kernel");
     });
 }

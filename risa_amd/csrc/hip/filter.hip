@@ -153,8 +153,8 @@ __global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict
     }
 }
 
-// Gradient w.r.t. the taps: for each row pair slot the complex row is
-// z = x + i g (one row); X[k] = (Z[k] + conj Z[D-k]) / 2,
+// Gradient w.r.t. the taps: for each row slot the complex row is
+// z = x + i sigma g (one row); X[k] = (Z[k] + conj Z[D-k]) / 2,
 // G[k] = (Z[k] - conj Z[D-k]) / 2i, contribution Re(conj(G[k]) X[k]).
 // Per-workgroup partial sums -> partial[blockIdx.x][k].
 __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restrict__ x, const float* __restrict__ g,
@@ -170,8 +170,30 @@ __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restr
     float2* b0 = lds + (size_t)slot * 2 * D;
     float2* b1 = b0 + D;
     const bool active = r < rows;
+    // x and g can differ in magnitude by many orders (g is a loss gradient):
+    // g is scaled by a power of two sigma that brings its row maximum to x's,
+    // or the rounding of X would swamp G in the shared complex transform
+    __shared__ unsigned int amax[2][2];
+    if (threadIdx.x < 4) amax[threadIdx.x / 2][threadIdx.x % 2] = 0u;
+    __syncthreads();
+    float mx = 0.0f, mg = 0.0f;
     if (active)
-        for (int n = t0; n < D; n += T) b0[n] = make_float2(x[r * D + n], g[r * D + n]);
+        for (int n = t0; n < D; n += T) {
+            const float xv = x[r * D + n], gv = g[r * D + n];
+            b0[n] = make_float2(xv, gv);
+            mx = fmaxf(mx, fabsf(xv));
+            mg = fmaxf(mg, fabsf(gv));
+        }
+    atomicMax(&amax[slot][0], __float_as_uint(mx));
+    atomicMax(&amax[slot][1], __float_as_uint(mg));
+    __syncthreads();
+    int e = 0;
+    {
+        const float ax = __uint_as_float(amax[slot][0]), ag = __uint_as_float(amax[slot][1]);
+        if (ax > 0.0f && ag > 0.0f) e = max(-100, min(100, ilogbf(ax) - ilogbf(ag)));
+    }
+    if (active && e != 0)
+        for (int n = t0; n < D; n += T) b0[n].y = ldexpf(b0[n].y, e);
     __syncthreads();
     float2* src = b0;
     float2* dst = b1;
@@ -190,7 +212,7 @@ __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restr
             const float2 zm = src[(D - k) % D];
             const float2 X = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             const float2 G = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-            c = G.x * X.x + G.y * X.y;
+            c = ldexpf(G.x * X.x + G.y * X.y, -e);
         }
         dst[k].x = c;
     }
