@@ -115,7 +115,8 @@ def main():
     ap.add_argument("--frames", type=int, default=2000, help="frames per plane in one step")
     ap.add_argument("--unique", type=int, default=16, help="distinct seeded frames per plane")
     ap.add_argument("--chunk", type=int, default=32, help="frames per chain call (graph)")
-    ap.add_argument("--cpu-frames", type=int, default=16, help="frame-planes in the CPU baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=256,
+                    help="frame-planes in the CPU baseline sample (about 10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
@@ -265,8 +266,9 @@ def main():
 
         oc = orc.Chain(g, avg_ref, avg_dark)
         n = args.cpu_frames
-        sample_raw = np.ascontiguousarray(uniq_raw[:n])
-        sample_planes = uniq_planes[:n]
+        reps_c = (n + len(uniq_planes) - 1) // len(uniq_planes)
+        sample_raw = np.ascontiguousarray(np.tile(uniq_raw, (reps_c, 1, 1))[:n])
+        sample_planes = np.ascontiguousarray(np.tile(uniq_planes, reps_c)[:n])
         threads = min(16, os.cpu_count() or 1)
         oc.f32(sample_raw[:1], sample_planes[:1], threads=1)
         t = time.perf_counter()

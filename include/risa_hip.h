@@ -109,6 +109,13 @@ size_t risa_geom_size(void);
  * (ConfigReader.h:41-90).  Missing keys -> RISA_ERR_CONFIG. */
 int risa_geom_from_config(const char* path, risa_geom* out);
 
+/* Stage keys beyond the geometry (memPoolSize_*, referenceInputPath,
+ * numberOfReferenceFrames, ...), read as the stages' readConfig does through
+ * ConfigReader::lookupValue (risaLib/include/risa/ConfigReader/ConfigReader.h:41-90;
+ * e.g. Attenuation.cu:387-408).  A missing key is RISA_ERR_CONFIG. */
+int risa_config_lookup_int(const char* path, const char* key, int* value);
+int risa_config_lookup_string(const char* path, const char* key, char* value, size_t capacity);
+
 /* ------------------------------------------------- host-side init tables -- */
 /* Relevant-area mask [P_f][D_f] (Attenuation::relevantAreaMask,
  * Attenuation.cu:327-381). */

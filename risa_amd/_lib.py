@@ -137,6 +137,8 @@ FP = C.POINTER(C.c_float)
 _SIGS = {
     "risa_abi_version": [],
     "risa_geom_from_config": [C.c_char_p, C.POINTER(Geom)],
+    "risa_config_lookup_int": [C.c_char_p, C.c_char_p, C.POINTER(C.c_int)],
+    "risa_config_lookup_string": [C.c_char_p, C.c_char_p, P, C.c_size_t],
     "risa_attenuation_area_mask": [C.POINTER(Geom), P],
     "risa_attenuation_prepare_reference": [C.POINTER(Geom), P, C.c_int, P, P],
     "risa_find_defect_detectors": [P, C.c_int, C.c_int, C.c_double, C.c_double, P],

@@ -1,0 +1,218 @@
+// The reconstruction pipeline as RISA/example/main.cpp:95-134 wires it:
+//   Source -> H2D -> Attenuation -> Fan2Para -> Filter -> Backprojection
+//          [-> Masking] -> D2H -> Sink
+// with frames streamed host -> device -> host through the stage queues.
+//
+// The source replays seeded synthetic uint16 fan frames held in pinned host
+// memory (the role of OfflineLoader_perfTest.cu:59-132, which replays
+// recorded frames from RAM); the sink counts slices, measures latency and
+// keeps the first few slices for checking.
+//
+//   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking]
+//
+// Prints one JSON line: frames, seconds, slices/s, mean and max latency.
+// --save writes K slices as records {int32 index, int32 frame, int32 plane,
+// float img[N*N]} (frame = the seeded frame id, see below).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include <glados/pipeline/Pipeline.h>
+#include <glados/pipeline/SinkStage.h>
+#include <glados/pipeline/SourceStage.h>
+#include <glados/pipeline/Stage.h>
+
+#include "risa/Stages.h"
+#include "risa_synth.h"
+
+namespace {
+
+// Frame id u of plane k is seeded as phantom 1000 + u, noise 55 + u*planes + k.
+struct Provenance {
+    std::mutex m;
+    std::map<std::size_t, std::pair<int, int>> of_index;   // index -> (frame id, plane)
+};
+
+class SyntheticLoader {
+public:
+    using manager_type = glados::hip::HostMemoryManager<uint16_t>;
+
+    SyntheticLoader(const std::string& configFile, long frames, int unique, Provenance* prov)
+        : frames_(frames), prov_(prov)
+    {
+        g_ = risa::read_geometry(configFile);
+        const int planes = g_.numberOfPlanes;
+        const std::size_t n = (std::size_t)g_.numberOfFanDetectors * g_.numberOfFanProjections;
+        auto* pool = glados::MemoryPool<manager_type>::instance();
+        const int count = unique * planes;
+        pool_ = pool->registerStage((std::size_t)count, n, 0);
+        risa_synth_opts o;
+        risa_synth_default_opts(&o);
+        // every pooled buffer holds one distinct frame for good; handing a
+        // buffer out replays the frame it holds
+        std::vector<glados::Image<manager_type>> held;
+        for (int b = 0; b < count; b++) {
+            auto img = pool->requestMemory(pool_);
+            const int u = b / planes, k = b % planes;
+            risa::check(risa_synth_fan_frame(&g_, k, 1000ull + (uint64_t)u, 55ull + (uint64_t)(u * planes + k), &o,
+                                             img.data()),
+                        "risa_synth_fan_frame");
+            ids_[img.data()] = {u, k};
+            held.push_back(std::move(img));
+        }
+    }
+
+    glados::Image<manager_type> loadImage()
+    {
+        if (next_ >= frames_) return glados::Image<manager_type>();
+        auto img = glados::MemoryPool<manager_type>::instance()->requestMemory(pool_);
+        const auto id = ids_.at(img.data());
+        img.setIdx((std::size_t)next_);
+        img.setPlane((std::size_t)id.second);
+        img.setStart(std::chrono::high_resolution_clock::now());
+        {
+            std::lock_guard<std::mutex> lock(prov_->m);
+            prov_->of_index[(std::size_t)next_] = id;
+        }
+        ++next_;
+        return img;
+    }
+
+private:
+    risa_geom g_{};
+    long frames_ = 0, next_ = 0;
+    int pool_ = -1;
+    Provenance* prov_;
+    std::map<const uint16_t*, std::pair<int, int>> ids_;
+};
+
+struct SinkState {
+    std::mutex m;
+    long count = 0;
+    double lat_sum = 0.0, lat_max = 0.0;
+    std::chrono::high_resolution_clock::time_point last;
+    int keep = 0;
+    std::map<std::size_t, std::vector<float>> kept;
+};
+
+class StatsSaver {
+public:
+    using manager_type = glados::hip::HostMemoryManager<float>;
+
+    explicit StatsSaver(SinkState* st) : st_(st) {}
+
+    void saveImage(glados::Image<manager_type>&& img, const std::string&)
+    {
+        const double lat = img.duration();
+        std::lock_guard<std::mutex> lock(st_->m);
+        st_->count++;
+        st_->lat_sum += lat;
+        st_->lat_max = std::max(st_->lat_max, lat);
+        st_->last = std::chrono::high_resolution_clock::now();
+        if ((long)img.index() < st_->keep) st_->kept[img.index()] = std::vector<float>(img.data(), img.data() + img.size());
+    }
+
+private:
+    SinkState* st_;
+};
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking]\n", argv[0]);
+        return 2;
+    }
+    const std::string config = argv[1];
+    long frames = 200;
+    int unique = 8, keep = 0;
+    bool masking = false;
+    std::string save_path;
+    for (int i = 2; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "--frames" && i + 1 < argc) frames = std::atol(argv[++i]);
+        else if (a == "--unique" && i + 1 < argc) unique = std::atoi(argv[++i]);
+        else if (a == "--save" && i + 2 < argc) {
+            keep = std::atoi(argv[++i]);
+            save_path = argv[++i];
+        } else if (a == "--masking") masking = true;
+        else {
+            std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+            return 2;
+        }
+    }
+
+    using namespace glados::pipeline;
+    using Source = SourceStage<SyntheticLoader>;
+    using Sink = SinkStage<StatsSaver>;
+
+    Provenance prov;
+    SinkState st;
+    st.keep = keep;
+    try {
+        Pipeline pipeline;
+        auto source = pipeline.create<Source>(config, frames, unique, &prov);
+        auto h2d = pipeline.create<Stage<risa::H2D>>(config);
+        auto attenuation = pipeline.create<Stage<risa::Attenuation>>(config);
+        auto fan2para = pipeline.create<Stage<risa::Fan2Para>>(config);
+        auto filter = pipeline.create<Stage<risa::Filter>>(config);
+        auto backprojection = pipeline.create<Stage<risa::Backprojection>>(config);
+        auto masking_stage = masking ? pipeline.create<Stage<risa::Masking>>(config) : nullptr;
+        auto d2h = pipeline.create<Stage<risa::D2H>>(config);
+        auto sink = pipeline.create<Sink>("", "IMG", &st);
+
+        pipeline.connect(source, h2d);
+        pipeline.connect(h2d, attenuation);
+        pipeline.connect(attenuation, fan2para);
+        pipeline.connect(fan2para, filter);
+        pipeline.connect(filter, backprojection);
+        if (masking_stage) {
+            pipeline.connect(backprojection, masking_stage);
+            pipeline.connect(masking_stage, d2h);
+        } else {
+            pipeline.connect(backprojection, d2h);
+        }
+        pipeline.connect(d2h, sink);
+
+        const auto t0 = std::chrono::high_resolution_clock::now();
+        if (masking_stage)
+            pipeline.run(source, h2d, attenuation, fan2para, filter, backprojection, masking_stage, d2h, sink);
+        else
+            pipeline.run(source, h2d, attenuation, fan2para, filter, backprojection, d2h, sink);
+        pipeline.wait();
+        const auto t1 = std::chrono::high_resolution_clock::now();
+        const double secs = std::chrono::duration<double>(t1 - t0).count();
+
+        if (!save_path.empty()) {
+            std::FILE* f = std::fopen(save_path.c_str(), "wb");
+            if (!f) throw std::runtime_error("cannot open " + save_path);
+            for (auto& kv : st.kept) {
+                const auto id = prov.of_index.at(kv.first);
+                const int32_t hdr[3] = {(int32_t)kv.first, id.first, id.second};
+                std::fwrite(hdr, sizeof(hdr), 1, f);
+                std::fwrite(kv.second.data(), sizeof(float), kv.second.size(), f);
+            }
+            std::fclose(f);
+        }
+        std::printf("{\"frames\": %ld, \"seconds\": %.4f, \"slices_per_s\": %.1f, \"latency_ms_mean\": %.3f, "
+                    "\"latency_ms_max\": %.3f, \"saved\": %zu}\n",
+                    st.count, secs, st.count / secs, st.count ? st.lat_sum / st.count : 0.0, st.lat_max,
+                    st.kept.size());
+        if (st.count != frames) {
+            std::fprintf(stderr, "expected %ld slices, got %ld\n", frames, st.count);
+            return 1;
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "risa_pipeline: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

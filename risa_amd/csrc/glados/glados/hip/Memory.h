@@ -1,0 +1,96 @@
+// HIP memory managers for Images: device memory (hipMalloc on the Image's
+// device) and pinned host memory (hipHostMalloc, the staging side of H2D/D2H
+// copies).  Roles as glados/cuda/DeviceMemoryManager.h and
+// HostMemoryManager.h; copies are plain hipMemcpy (stages do their own
+// asynchronous copies on their streams).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+namespace glados {
+namespace hip {
+
+inline void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Sets the device for the lifetime of the object, restoring the previous one.
+class ScopedDevice {
+public:
+    explicit ScopedDevice(int d)
+    {
+        check(hipGetDevice(&prev_), "hipGetDevice");
+        if (prev_ != d) check(hipSetDevice(d), "hipSetDevice");
+    }
+    ~ScopedDevice() { (void)hipSetDevice(prev_); }
+
+private:
+    int prev_ = 0;
+};
+
+template <class T>
+class DeviceMemoryManager {
+public:
+    using value_type = T;
+    using size_type = std::size_t;
+    struct Deleter {
+        void operator()(T* p) const noexcept { (void)hipFree(p); }
+    };
+    using pointer_type_1D = std::unique_ptr<T[], Deleter>;
+
+    int device() const noexcept { return device_; }
+    void setDevice(int d) noexcept { device_ = d; }
+
+    pointer_type_1D make_ptr(size_type n) const
+    {
+        ScopedDevice sd(device_);
+        void* p = nullptr;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw std::bad_alloc();
+        return pointer_type_1D(static_cast<T*>(p));
+    }
+    void copy(pointer_type_1D& dst, const pointer_type_1D& src, size_type n) const
+    {
+        check(hipMemcpy(dst.get(), src.get(), n * sizeof(T), hipMemcpyDefault), "hipMemcpy");
+    }
+
+protected:
+    int device_ = 0;
+};
+
+template <class T>
+class HostMemoryManager {
+public:
+    using value_type = T;
+    using size_type = std::size_t;
+    struct Deleter {
+        void operator()(T* p) const noexcept { (void)hipHostFree(p); }
+    };
+    using pointer_type_1D = std::unique_ptr<T[], Deleter>;
+
+    int device() const noexcept { return device_; }
+    void setDevice(int d) noexcept { device_ = d; }
+
+    pointer_type_1D make_ptr(size_type n) const
+    {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return pointer_type_1D(static_cast<T*>(p));
+    }
+    void copy(pointer_type_1D& dst, const pointer_type_1D& src, size_type n) const
+    {
+        check(hipMemcpy(dst.get(), src.get(), n * sizeof(T), hipMemcpyDefault), "hipMemcpy");
+    }
+
+protected:
+    int device_ = 0;
+};
+
+}  // namespace hip
+}  // namespace glados

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "common.h"
+#include "config.h"
 #include "tables.h"
 
 namespace risa {
@@ -47,6 +48,27 @@ const char* risa_last_error(void) { return g_last_error.c_str(); }
 int risa_abi_version(void) { return 1; }
 
 size_t risa_geom_size(void) { return sizeof(risa_geom); }
+
+int risa_config_lookup_int(const char* path, const char* key, int* value)
+{
+    return guarded("risa_config_lookup_int", [&] {
+        if (!path || !key || !value) fail(RISA_ERR_ARG, "null argument");
+        const ConfigReader cfg(path);
+        if (!cfg.lookupValue(key, *value)) fail(RISA_ERR_CONFIG, "missing or non-integer key '%s'", key);
+    });
+}
+
+int risa_config_lookup_string(const char* path, const char* key, char* value, size_t capacity)
+{
+    return guarded("risa_config_lookup_string", [&] {
+        if (!path || !key || !value || capacity == 0) fail(RISA_ERR_ARG, "null argument");
+        const ConfigReader cfg(path);
+        std::string v;
+        if (!cfg.lookupValue(key, v)) fail(RISA_ERR_CONFIG, "missing or non-string key '%s'", key);
+        if (v.size() + 1 > capacity) fail(RISA_ERR_ARG, "value of '%s' longer than the buffer", key);
+        std::memcpy(value, v.c_str(), v.size() + 1);
+    });
+}
 
 int risa_attenuation_area_mask(const risa_geom* g, float* mask)
 {

@@ -1,0 +1,287 @@
+// Stage implementations over the C-ABI (see risa/Stages.h).
+#include "risa/Stages.h"
+
+#include <cstdio>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+
+#include "risa_synth.h"
+
+namespace risa {
+
+void check(int rc, const char* what)
+{
+    if (rc != RISA_OK)
+        throw std::runtime_error(std::string("CUDA/HIP error in ") + what + ": " + risa_last_error());
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int config_int(const std::string& path, const char* key)
+{
+    int v = 0;
+    check(risa_config_lookup_int(path.c_str(), key, &v), key);
+    return v;
+}
+
+int config_int_or(const std::string& path, const char* key, int fallback)
+{
+    int v = 0;
+    return risa_config_lookup_int(path.c_str(), key, &v) == RISA_OK ? v : fallback;
+}
+
+std::string config_string_or(const std::string& path, const char* key, const std::string& fallback)
+{
+    char buf[4096];
+    return risa_config_lookup_string(path.c_str(), key, buf, sizeof(buf)) == RISA_OK ? std::string(buf) : fallback;
+}
+
+int device_count()
+{
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (n < 1) throw std::runtime_error("no HIP device visible");
+    return n;
+}
+
+risa_geom read_geometry(const std::string& configFile)
+{
+    risa_geom g{};
+    check(risa_geom_from_config(configFile.c_str(), &g), "readConfig");
+    return g;
+}
+
+static std::size_t fan_elems(const risa_geom& g)
+{
+    return (std::size_t)g.numberOfFanDetectors * g.numberOfFanProjections;
+}
+static std::size_t par_elems(const risa_geom& g)
+{
+    return (std::size_t)g.numberOfParallelDetectors * g.numberOfParallelProjections;
+}
+static std::size_t img_elems(const risa_geom& g) { return (std::size_t)g.numberOfPixels * g.numberOfPixels; }
+
+// ------------------------------------------------------- reference frames --
+std::vector<uint16_t> read_module_frames(const std::string& stem, const risa_geom& g, int frames)
+{
+    const int modules = g.numberOfDetectorModules, D = g.numberOfFanDetectors, P = g.numberOfFanProjections;
+    const int planes = g.numberOfPlanes;
+    if (modules <= 0 || D % modules) throw std::runtime_error("numberOfDetectorModules must divide the detectors");
+    const int per = D / modules;
+    std::vector<uint16_t> out((std::size_t)frames * planes * P * D);
+    for (int m = 0; m < modules; m++) {
+        const std::string path = stem + std::to_string(m + 1) + ".fx";
+        std::ifstream in(path, std::ios::binary);
+        if (!in) throw std::runtime_error("Attenuation: reference file " + path + " could not be loaded");
+        std::vector<uint16_t> content((std::size_t)frames * planes * P * per);
+        in.read(reinterpret_cast<char*>(content.data()), (std::streamsize)(content.size() * sizeof(uint16_t)));
+        if (!in) throw std::runtime_error("Attenuation: reference file " + path + " is too short");
+        for (int b = 0; b < frames * planes; b++)
+            for (int p = 0; p < P; p++) {
+                const uint16_t* src = content.data() + ((std::size_t)b * P + p) * per;
+                uint16_t* dst = out.data() + ((std::size_t)b * P + p) * D + (std::size_t)m * per;
+                std::copy(src, src + per, dst);
+            }
+    }
+    return out;
+}
+
+void write_module_frames(const std::string& stem, const risa_geom& g, int frames, const std::vector<uint16_t>& data)
+{
+    const int modules = g.numberOfDetectorModules, D = g.numberOfFanDetectors, P = g.numberOfFanProjections;
+    const int planes = g.numberOfPlanes, per = D / modules;
+    for (int m = 0; m < modules; m++) {
+        std::vector<uint16_t> content((std::size_t)frames * planes * P * per);
+        for (int b = 0; b < frames * planes; b++)
+            for (int p = 0; p < P; p++) {
+                const uint16_t* src = data.data() + ((std::size_t)b * P + p) * D + (std::size_t)m * per;
+                std::copy(src, src + per, content.data() + ((std::size_t)b * P + p) * per);
+            }
+        std::ofstream out(stem + std::to_string(m + 1) + ".fx", std::ios::binary);
+        out.write(reinterpret_cast<const char*>(content.data()), (std::streamsize)(content.size() * sizeof(uint16_t)));
+        if (!out) throw std::runtime_error("could not write module file");
+    }
+}
+
+std::vector<uint16_t> reference_frames(const std::string& configFile, const risa_geom& g, int R)
+{
+    std::string dir = config_string_or(configFile, "referenceInputPath", "");
+    if (!dir.empty()) {
+        if (dir.back() != '/') dir.push_back('/');
+        return read_module_frames(dir + "ref_empty_tomograph_repaired_DetModNr_", g, R);
+    }
+    risa_synth_opts o;
+    risa_synth_default_opts(&o);
+    const uint64_t seed = (uint64_t)config_int_or(configFile, "synthReferenceSeed", 901);
+    std::vector<uint16_t> out((std::size_t)R * g.numberOfPlanes * fan_elems(g));
+    for (int b = 0; b < R * g.numberOfPlanes; b++)
+        check(risa_synth_reference_frame(&g, seed * 1000003ull + (uint64_t)b + 1, &o, out.data() + b * fan_elems(g)),
+              "risa_synth_reference_frame");
+    return out;
+}
+
+// ------------------------------------------------------------------ H2D --
+H2D::H2D(const std::string& configFile)
+{
+    const risa_geom g = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_H2D");
+    const std::size_t n = fan_elems(g);
+    route_by_index_ = true;
+    start("H2D", device_count(), pool, n, [n](int, input_type& in, output_type& out, hipStream_t s) {
+        if (in.size() < n) throw std::runtime_error("H2D: frame smaller than the fan sinogram");
+        hip_check(hipMemcpyAsync(out.data(), in.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync");
+    });
+}
+H2D::~H2D() { stop(); }
+
+// ---------------------------------------------------------- Attenuation --
+Attenuation::Attenuation(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int R = config_int(configFile, "numberOfReferenceFrames");
+    std::vector<uint16_t> frames = reference_frames(configFile, g_, R);
+    std::vector<float> ref((std::size_t)g_.numberOfPlanes * fan_elems(g_));
+    std::vector<float> dark((std::size_t)g_.numberOfPlanes * g_.numberOfFanDetectors);
+    check(risa_attenuation_prepare_reference(&g_, frames.data(), R, ref.data(), dark.data()), "Attenuation::init");
+    init(configFile, ref, dark);
+}
+
+Attenuation::Attenuation(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark)
+{
+    g_ = read_geometry(configFile);
+    if (ref.size() != (std::size_t)g_.numberOfPlanes * fan_elems(g_) ||
+        dark.size() != (std::size_t)g_.numberOfPlanes * g_.numberOfFanDetectors)
+        throw std::runtime_error("Attenuation: reference/dark averages have the wrong size");
+    init(configFile, ref, dark);
+}
+
+void Attenuation::init(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark)
+{
+    const int pool = config_int(configFile, "memPoolSize_attenuation");
+    const int devices = device_count();
+    for (int d = 0; d < devices; d++) {
+        risa_attenuation_t h = nullptr;
+        check(risa_hip_attenuation_create(&g_, ref.data(), dark.data(), d, &h), "Attenuation");
+        h_.push_back(h);
+    }
+    start("Attenuation", devices, pool, fan_elems(g_), [this](int d, input_type& in, output_type& out, hipStream_t s) {
+        check(risa_hip_attenuation_run(h_[(std::size_t)d], in.data(), (int)in.plane(), out.data(), s),
+              "Attenuation::processor");
+    });
+}
+
+Attenuation::~Attenuation()
+{
+    stop();
+    for (auto h : h_) risa_hip_attenuation_destroy(h);
+}
+
+// ------------------------------------------------------------- Fan2Para --
+Fan2Para::Fan2Para(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_fan2Para");
+    const int devices = device_count();
+    for (int d = 0; d < devices; d++) {
+        risa_fan2para_t h = nullptr;
+        check(risa_hip_fan2para_create(&g_, d, &h), "Fan2Para");
+        h_.push_back(h);
+    }
+    start("Fan2Para", devices, pool, par_elems(g_), [this](int d, input_type& in, output_type& out, hipStream_t s) {
+        check(risa_hip_fan2para_run(h_[(std::size_t)d], in.data(), (int)in.plane(), out.data(), s),
+              "Fan2Para::processor");
+    });
+}
+
+Fan2Para::~Fan2Para()
+{
+    stop();
+    for (auto h : h_) risa_hip_fan2para_destroy(h);
+}
+
+// --------------------------------------------------------------- Filter --
+Filter::Filter(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int devices = device_count();
+    for (int d = 0; d < devices; d++) {
+        risa_filter_t h = nullptr;
+        check(risa_hip_filter_create(&g_, d, RISA_FILTER_MODE_FFT32, &h), "Filter");
+        h_.push_back(h);
+    }
+    start("Filter", devices, 0, 0, [this](int d, input_type& img, output_type&, hipStream_t s) {
+        check(risa_hip_filter_run(h_[(std::size_t)d], img.data(), 1, s), "Filter::processor");
+    });
+}
+
+Filter::~Filter()
+{
+    stop();
+    for (auto h : h_) risa_hip_filter_destroy(h);
+}
+
+// ------------------------------------------------------- Backprojection --
+Backprojection::Backprojection(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_backProjection");
+    const int devices = device_count();
+    for (int d = 0; d < devices; d++) {
+        risa_bp_t h = nullptr;
+        check(risa_hip_bp_create(&g_, d, &h), "Backprojection");
+        h_.push_back(h);
+        glados::hip::ScopedDevice sd(d);
+        check(risa_hip_bp_autotune(h, nullptr), "Backprojection autotune");
+    }
+    start("Backprojection", devices, pool, img_elems(g_),
+          [this](int d, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_bp_run(h_[(std::size_t)d], in.data(), out.data(), s), "Backprojection::processor");
+          });
+}
+
+Backprojection::~Backprojection()
+{
+    stop();
+    for (auto h : h_) risa_hip_bp_destroy(h);
+}
+
+// -------------------------------------------------------------- Masking --
+Masking::Masking(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int devices = device_count();
+    for (int d = 0; d < devices; d++) {
+        risa_mask_t h = nullptr;
+        check(risa_hip_mask_create(&g_, d, &h), "Masking");
+        h_.push_back(h);
+    }
+    start("Masking", devices, 0, 0, [this](int d, input_type& img, output_type&, hipStream_t s) {
+        check(risa_hip_mask_run(h_[(std::size_t)d], img.data(), s), "Masking::processor");
+    });
+}
+
+Masking::~Masking()
+{
+    stop();
+    for (auto h : h_) risa_hip_mask_destroy(h);
+}
+
+// ------------------------------------------------------------------ D2H --
+D2H::D2H(const std::string& configFile)
+{
+    const risa_geom g = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_D2H");
+    const std::size_t n = img_elems(g);
+    start("D2H", device_count(), pool, n, [n](int, input_type& in, output_type& out, hipStream_t s) {
+        hip_check(hipMemcpyAsync(out.data(), in.data(), n * sizeof(float), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync");
+    });
+}
+D2H::~D2H() { stop(); }
+
+}  // namespace risa

@@ -1,0 +1,203 @@
+// Shared machinery of the GPU stages: per-device processor threads, each
+// with its own HIP stream (default priority) and input queue, an output
+// memory pool per device for allocating stages, and the results queue that
+// wait() drains.
+//
+// The protocol is the reference stages' (e.g. Backprojection.cu:91-208):
+// process() hands a valid Image to the queue of its device; the invalid
+// sentinel makes every processor thread finish its queue and exit, after
+// which the sentinel is put on the results queue.  Each frame's launches are
+// stream-ordered and the stream is synchronised before the frame is handed
+// on, so an output Image is complete when wait() returns it.
+//
+// Errors: a C-ABI status is turned into std::runtime_error with the
+// library's message (the CHECK format of the reference).  An error on a
+// processor thread ends that thread, is kept, and is rethrown by wait().
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <exception>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include <glados/Image.h>
+#include <glados/Queue.h>
+#include <glados/hip/Memory.h>
+
+#include "risa_hip.h"
+
+namespace risa {
+
+// Throws std::runtime_error for a non-zero C-ABI status.
+void check(int rc, const char* what);
+// Throws std::runtime_error for a HIP error.
+void hip_check(hipError_t e, const char* what);
+
+// Stage keys: integer / string lookups that throw on a missing key, and the
+// optional form with a default.
+int config_int(const std::string& path, const char* key);
+int config_int_or(const std::string& path, const char* key, int fallback);
+std::string config_string_or(const std::string& path, const char* key, const std::string& fallback);
+
+// Number of visible devices the stages spread frames over.
+int device_count();
+
+template <class InT, class OutT>
+class DeviceStage {
+public:
+    using input_type = InT;
+    using output_type = OutT;
+
+    DeviceStage(const DeviceStage&) = delete;
+    DeviceStage& operator=(const DeviceStage&) = delete;
+    // Derived stages call stop() first (their handles must outlive the
+    // processor threads); this only catches a stage that did not.
+    virtual ~DeviceStage()
+    {
+        if (!workers_.empty()) stop();
+    }
+
+    void process(input_type&& img)
+    {
+        if (img.valid()) {
+            // frames from the host are dealt round-robin by index (H2D.cu:92-93);
+            // device frames stay on their device
+            const int d = (int)((route_by_index_ ? (int)img.index() : img.device()) % (int)workers_.size());
+            workers_[(size_t)d]->queue.push(std::move(img));
+            return;
+        }
+        finish();
+    }
+
+    output_type wait()
+    {
+        output_type out = results_.take();
+        if (!out.valid()) rethrow();
+        return out;
+    }
+
+protected:
+    using compute_fn = std::function<void(int dev, input_type& in, output_type& out, hipStream_t s)>;
+
+    DeviceStage() = default;
+
+    // Starts one processor thread per device.  With out_elements > 0 the
+    // stage allocates: each device gets a pool of pool_size output buffers
+    // and fn writes into a fresh one; with 0 the stage works in place and
+    // forwards the input Image itself (input_type must equal output_type).
+    void start(const char* name, int devices, int pool_size, std::size_t out_elements, compute_fn fn)
+    {
+        name_ = name;
+        fn_ = std::move(fn);
+        allocating_ = out_elements > 0;
+        for (int d = 0; d < devices; d++) {
+            auto w = std::unique_ptr<Worker>(new Worker);
+            w->device = d;
+            glados::hip::ScopedDevice sd(d);
+            hip_check(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            if (allocating_)
+                w->pool = glados::MemoryPool<typename output_type::manager_type>::instance()->registerStage(
+                    (std::size_t)pool_size, out_elements, d);
+            workers_.push_back(std::move(w));
+        }
+        for (auto& w : workers_) w->thread = std::thread(&DeviceStage::loop, this, w.get());
+    }
+
+    // Joins the processor threads (the sentinel is sent if process() never
+    // saw one) and releases streams and pools.  Call from the derived
+    // destructor, before its handles go.
+    void stop()
+    {
+        if (!stopped_) finish();
+        for (auto& w : workers_) {
+            if (w->stream) (void)hipStreamDestroy(w->stream);
+            if (allocating_ && w->pool >= 0)
+                glados::MemoryPool<typename output_type::manager_type>::instance()->freeMemory(w->pool);
+        }
+        workers_.clear();
+    }
+
+private:
+    struct Worker {
+        int device = 0;
+        hipStream_t stream = nullptr;
+        int pool = -1;
+        glados::Queue<input_type> queue;
+        std::thread thread;
+    };
+
+    void finish()
+    {
+        std::lock_guard<std::mutex> lock(finish_m_);
+        if (stopped_) return;
+        for (auto& w : workers_) w->queue.push(input_type());
+        for (auto& w : workers_)
+            if (w->thread.joinable()) w->thread.join();
+        stopped_ = true;
+        results_.push(output_type());
+    }
+
+    void loop(Worker* w)
+    {
+        try {
+            glados::hip::ScopedDevice sd(w->device);
+            for (;;) {
+                input_type in = w->queue.take();
+                if (!in.valid()) break;
+                if constexpr (std::is_same<input_type, output_type>::value) {
+                    if (!allocating_) {
+                        fn_(w->device, in, in, w->stream);
+                        hip_check(hipStreamSynchronize(w->stream), name_);
+                        results_.push(std::move(in));
+                        continue;
+                    }
+                }
+                output_type out =
+                    glados::MemoryPool<typename output_type::manager_type>::instance()->requestMemory(w->pool);
+                fn_(w->device, in, out, w->stream);
+                hip_check(hipStreamSynchronize(w->stream), name_);
+                out.setIdx(in.index());
+                out.setPlane(in.plane());
+                out.setStart(in.start());
+                results_.push(std::move(out));
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(err_m_);
+            if (!error_) error_ = std::current_exception();
+            // drain so producers never block on a dead consumer
+            for (;;) {
+                input_type in = w->queue.take();
+                if (!in.valid()) break;
+            }
+        }
+    }
+
+    void rethrow()
+    {
+        std::lock_guard<std::mutex> lock(err_m_);
+        if (error_) std::rethrow_exception(error_);
+    }
+
+protected:
+    bool route_by_index_ = false;
+
+private:
+    const char* name_ = "stage";
+    compute_fn fn_;
+    bool allocating_ = false;
+    bool stopped_ = false;
+    std::mutex finish_m_, err_m_;
+    std::exception_ptr error_;
+    std::vector<std::unique_ptr<Worker>> workers_;
+    glados::Queue<output_type> results_;
+};
+
+}  // namespace risa

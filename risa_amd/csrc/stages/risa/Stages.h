@@ -1,0 +1,116 @@
+// The reconstruction stages as glados pipeline stages.  Each keeps the
+// reference class's shape -- input_type / output_type, a constructor taking
+// the libconfig file, process(input_type&&), wait() -> output_type -- and
+// reads the same keys; the work is done by the HIP kernels behind the C-ABI
+// (include/risa_hip.h).
+//
+//   H2D            Copy/H2D.cu:84-169            host uint16 frame -> device
+//   Attenuation    Attenuation.cu:48-175         uint16 fan sinogram -> float
+//   Fan2Para       Fan2Para.cu:56-249            fan -> parallel sinogram
+//   Filter         Filter.cu:54-143              in place
+//   Backprojection Backprojection.cu:46-208      parallel sinogram -> slice
+//   Masking        Masking.cu:46-169             in place
+//   D2H            Copy/D2H.cu:83-148            device slice -> host
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <glados/Image.h>
+#include <glados/hip/Memory.h>
+
+#include "DeviceStage.h"
+#include "risa_hip.h"
+
+namespace risa {
+
+template <class T>
+using DeviceImage = glados::Image<glados::hip::DeviceMemoryManager<T>>;
+template <class T>
+using HostImage = glados::Image<glados::hip::HostMemoryManager<T>>;
+
+// Geometry and filter keys of a configuration file (throws on error).
+risa_geom read_geometry(const std::string& configFile);
+
+// The R reference frames per plane the Attenuation stage averages: read from
+// the module files under referenceInputPath
+// (<path>ref_empty_tomograph_repaired_DetModNr_<m>.fx, m = 1..modules, each
+// module's columns for every frame, re-sorted as Attenuation.cu:302-321
+// does), or, when referenceInputPath is empty, synthesised with the seeded
+// generator (flat field Poisson(60000) + dark N(200, 5)).
+std::vector<uint16_t> reference_frames(const std::string& configFile, const risa_geom& g, int R);
+
+// The module-major file layout of the reference frames (Attenuation.cu:275-321):
+// module m holds, for each frame f, plane k and projection p, its
+// D/modules detector values.  Read (and the inverse, for tests).
+std::vector<uint16_t> read_module_frames(const std::string& stem, const risa_geom& g, int frames);
+void write_module_frames(const std::string& stem, const risa_geom& g, int frames, const std::vector<uint16_t>& data);
+
+class H2D : public DeviceStage<HostImage<uint16_t>, DeviceImage<uint16_t>> {
+public:
+    explicit H2D(const std::string& configFile);
+    ~H2D();
+};
+
+class Attenuation : public DeviceStage<DeviceImage<uint16_t>, DeviceImage<float>> {
+public:
+    explicit Attenuation(const std::string& configFile);
+    // With the averaged reference [planes][P_f][D_f] and dark [planes][D_f] given.
+    Attenuation(const std::string& configFile, const std::vector<float>& avgReference,
+                const std::vector<float>& avgDark);
+    ~Attenuation();
+
+private:
+    void init(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark);
+    risa_geom g_{};
+    std::vector<risa_attenuation_t> h_;
+};
+
+class Fan2Para : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
+public:
+    explicit Fan2Para(const std::string& configFile);
+    ~Fan2Para();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_fan2para_t> h_;
+};
+
+class Filter : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
+public:
+    explicit Filter(const std::string& configFile);
+    ~Filter();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_filter_t> h_;
+};
+
+class Backprojection : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
+public:
+    explicit Backprojection(const std::string& configFile);
+    ~Backprojection();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_bp_t> h_;
+};
+
+class Masking : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
+public:
+    explicit Masking(const std::string& configFile);
+    ~Masking();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_mask_t> h_;
+};
+
+class D2H : public DeviceStage<DeviceImage<float>, HostImage<float>> {
+public:
+    explicit D2H(const std::string& configFile);
+    ~D2H();
+};
+
+}  // namespace risa

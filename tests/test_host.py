@@ -162,3 +162,19 @@ def test_synth_phantom_statistics(risa):
         assert (b >= 0.05 * 95.0 - 1e-9).all() and (b <= 0.4 * 95.0 + 1e-9).all()
     img = synth.phantom_image(1, 190.0, 64)
     assert img.shape == (64, 64) and img.max() > 0 and img.min() >= 0
+
+
+def test_glados_framework_selftest(risa):
+    """The C++ pipeline framework and the stages' host side, run on the CPU:
+    queue bound and order, pool backpressure, sentinel propagation through a
+    Source -> Stage -> Stage -> Sink pipeline, the module-major reference file
+    layout (Attenuation.cu:275-321) and stage configuration errors."""
+    import subprocess
+
+    exe = ROOT / "risa_amd" / "bin" / "glados_selftest"
+    assert exe.exists(), "build with `make -C risa_amd`"
+    r = subprocess.run([str(exe), str(ROOT / "risa_amd" / "configs" / "c0.cfg")], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "checks passed" in r.stdout
