@@ -199,6 +199,13 @@ int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, void* stream);
 /* Out of place with FP16 output (value * out_scale stored as half). */
 int risa_hip_filter_run_f16(risa_filter_t h, const float* sino, uint16_t* out, int slices,
                             float out_scale, void* stream);
+/* DFT-GEMM mode: FP16 output (value * out_scale) in the "x4" layout the FP16
+ * backprojector reads: halves[((g*P + a)*D + k)*4 + b] = slice 4g+b, angle a,
+ * detector k; slices past the last one of the last group are written as zero.
+ * Buffer: ceil(slices/4)*P*D*4 halves, 8-byte aligned.  (No reference
+ * counterpart: the mixed-precision chain's layout.) */
+int risa_hip_filter_run_f16x4(risa_filter_t h, const float* sino, uint16_t* out_x4, int slices,
+                              float out_scale, void* stream);
 /* Adjoint of the filter (the circulant is symmetric: the same operator). */
 int risa_hip_filter_adjoint_run(risa_filter_t h, float* grad, int slices, void* stream);
 /* Gradient of a loss w.r.t. the D/2+1 taps given the unfiltered input x and
@@ -217,10 +224,19 @@ int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out);
  * (Backprojection.cu:238-336) as selected by interpolationType and
  * useTextureMemory.  sino [P][D] f32 -> img [N][N]. */
 int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void* stream);
+/* `slices` FP32 sinograms of this geometry ([slices][P][D] -> [slices][N][N])
+ * in one launch: the linear backprojector of risa_hip_bp_run with the slices
+ * as a grid dimension (no reference counterpart; the reference launches one
+ * slice per processor call, Backprojection.cu:190-200). */
+int risa_hip_bp_run_batch(risa_bp_t h, const float* sino, int slices, float* img, void* stream);
 /* `slices` sinograms of this geometry stored as FP16 (value * in_scale),
  * FP32 accumulation; img [slices][N][N].  Linear interpolation only. */
 int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_scale, int slices,
                         float* img, void* stream);
+/* The same on an FP16 sinogram already in the x4 layout written by
+ * risa_hip_filter_run_f16x4 (4 slices interleaved per detector sample). */
+int risa_hip_bp_run_f16x4(risa_bp_t h, const uint16_t* sino_x4, float in_scale, int slices,
+                          float* img, void* stream);
 /* Adjoint of risa_hip_bp_run (linear): the ray-driven projector that is its
  * exact transpose (mask included when fused).  img [N][N] -> sino [P][D]. */
 int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream);

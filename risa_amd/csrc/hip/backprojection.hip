@@ -129,7 +129,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     const int W = p.W, AE = p.AE;
     float2* rows = reinterpret_cast<float2*>(smem);
     float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(float2));
-    int2* kinfo = reinterpret_cast<int2*>(info + p.P);
+    int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int x = x0 + tx;
     const float xp = ((float)x - p.center) * p.scale;
@@ -157,23 +157,32 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
     for (int r = 0; r < PPT; r++) acc[r] = 0.0f;
 
-    {
-        const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
-        const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
-        for (int a = tid; a < p.P; a += NT) {
-            const AngleWin w = angle_window(cosT[a], sinT[a], xa, xb, ya, yb, p.cI);
-            const int row = ((a / AE) & 1) * AE * W + (a % AE) * W;
-            info[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * 8));
-            kinfo[a] = make_int2(w.kmin, w.kabs);
+    const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
+    const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
+    const int nch = (p.P + AE - 1) / AE;
+    // the windows of chunk ch go to ring slot ch % 3: chunk ch is computed
+    // while ch + 1 is fetched and ch + 2 is prepared, one barrier per chunk
+    auto make_info = [&](int ch) {
+        if (ch >= nch) return;
+        const int p0 = ch * AE, na = min(AE, p.P - p0);
+        float4* inf = info + (ch % 3) * AE;
+        int2* kin = kinfo + (ch % 3) * AE;
+        for (int a = tid; a < na; a += NT) {
+            const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
+            const int row = (ch & 1) * AE * W + a * W;
+            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * 8));
+            kin[a] = make_int2(w.kmin, w.kabs);
         }
-    }
+    };
+    make_info(0);
+    make_info(1);
     __syncthreads();
 
-    const int nch = (p.P + AE - 1) / AE;
     float s0r[E], s1r[E];
     int kr[E];
     auto fetch = [&](int ch) {
         const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
+        const int2* kin = kinfo + (ch % 3) * AE;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
@@ -183,7 +192,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
                 const int w = e - a * W;
-                const int2 ki = kinfo[p0 + a];
+                const int2 ki = kin[a];
                 const int k = ki.y + w;
                 kr[q] = ki.x + w;
                 const long off = (long)(p0 + a) * p.D;
@@ -216,6 +225,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     const char* rb = reinterpret_cast<const char*>(rows);
     for (int ch = 0; ch < nch; ch++) {
         const int p0 = ch * AE, na = min(AE, p.P - p0);
+        const float4* inf = info + (ch % 3) * AE;
         if (ch + 1 < nch) fetch(ch + 1);
         if (wave_active) {
             // U angles per step: all U*PPT LDS reads are issued before the
@@ -226,7 +236,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 float t[U][PPT];
                 float2 v[U][PPT];
 #pragma unroll
-                for (int u = 0; u < U; u++) in[u] = info[p0 + a + u];
+                for (int u = 0; u < U; u++) in[u] = inf[a + u];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
@@ -246,7 +256,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                     }
             }
             for (; a < na; a++) {
-                const float4 in = info[p0 + a];
+                const float4 in = inf[a];
                 const float xc = __builtin_fmaf(xp, in.x, in.z);
                 const int b = __float_as_int(in.w);
 #pragma unroll
@@ -259,6 +269,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             }
         }
         if (ch + 1 < nch) commit(ch + 1);
+        make_info(ch + 2);
         __syncthreads();
     }
 #pragma unroll
@@ -318,19 +329,48 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_persistent_kernel(const BpJob
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
-// FP16-sinogram variant over B slices of one geometry: each staged entry
-// holds, per slice, the pair (s[k], s[k+1] - s[k]) as two halves; an update
-// of all B slices is one LDS read of 4B bytes and B v_dot2_f32_f16 with the
-// weight pair (1, f).  Same tiling, windows and double buffering as bp_tile.
+// FP16-sinogram variant over groups of 4 slices of one geometry.
+//
+// Input layout "x4": for slice group g, angle a and detector k the four
+// slices' values are adjacent, halves[((g*P + a)*D + k)*4 + b] (value *
+// in_scale); slices past the last one of the last group are zero.  The DFT-GEMM
+// filter writes this layout directly.  Each staged LDS entry holds, per slice,
+// the pair (s[k], s[k+1] - s[k]); one update of the 4 slices is one 16-byte
+// LDS read, one v_cvt_pk_f16_f32 for the weight pair (1, f) and four
+// v_dot2_f32_f16 into FP32 accumulators.  Same tiling, windows and chunked
+// double buffering as bp_tile.
 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
 
-template <int B>
+// (1, f) as an FP16 pair in one v_cvt_pk_f16_f32 (round to nearest even).
+__device__ __forceinline__ h2_t weight_pair(float f)
+{
+    const f2_t v = {1.0f, f};
+    return __builtin_convertvector(v, h2_t);
+}
+
+constexpr int kF16Group = 4;
+
 struct HRow {
-    h2_t v[B];
+    h2_t v[kF16Group];
 };
 
-template <int TX, int TY, int PPT, int B, int E, int U>
-__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint16_t* __restrict__ sino,
+// (s[k], s[k+1]) of 4 slices (two 8-byte loads, zero outside [0, D)) ->
+// the staged pairs (s[k], s[k+1] - s[k]); the difference is one FP16 rounding.
+__device__ __forceinline__ HRow stage_pairs(uint2 q0, uint2 q1)
+{
+    const h2_t s01 = __builtin_bit_cast(h2_t, q0.x), s23 = __builtin_bit_cast(h2_t, q0.y);
+    const h2_t d01 = __builtin_bit_cast(h2_t, q1.x) - s01, d23 = __builtin_bit_cast(h2_t, q1.y) - s23;
+    HRow r;
+    r.v[0] = h2_t{s01.x, d01.x};
+    r.v[1] = h2_t{s01.y, d01.y};
+    r.v[2] = h2_t{s23.x, d23.x};
+    r.v[3] = h2_t{s23.y, d23.y};
+    return r;
+}
+
+template <int TX, int TY, int PPT, int E, int U>
+__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2* __restrict__ sino,
                                                                      float* __restrict__ img,
                                                                      const float* __restrict__ cosT,
                                                                      const float* __restrict__ sinT, int slices,
@@ -338,18 +378,20 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
 {
     constexpr int NT = TX * TY / PPT;
     constexpr int RS = TY / PPT;
+    constexpr int B = kF16Group;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int W = p.W, AE = p.AE;
-    HRow<B>* rows = reinterpret_cast<HRow<B>*>(smem);
-    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(HRow<B>));
-    int2* kinfo = reinterpret_cast<int2*>(info + p.P);
+    HRow* rows = reinterpret_cast<HRow*>(smem);
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(HRow));
+    int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
-    const int b0 = blockIdx.y * B;
+    const int g = blockIdx.y;
+    const int b0 = g * B;
     const int nb = min(B, slices - b0);
-    const long plane = (long)p.P * p.D;
+    const uint2* S = sino + (long)g * p.P * p.D;
 
     const int x = x0 + tx;
     const float xp = ((float)x - p.center) * p.scale;
@@ -379,50 +421,51 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
     for (int r = 0; r < PPT; r++)
 #pragma unroll
         for (int b = 0; b < B; b++) acc[r][b] = 0.0f;
-    {
-        const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
-        const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
-        for (int a = tid; a < p.P; a += NT) {
-            const AngleWin w = angle_window(cosT[a], sinT[a], xa, xb, ya, yb, p.cI);
-            const int row = ((a / AE) & 1) * AE * W + (a % AE) * W;
-            info[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow<B>)));
-            kinfo[a] = make_int2(w.kmin, w.kabs);
+
+    const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
+    const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
+    const int nch = (p.P + AE - 1) / AE;
+    auto make_info = [&](int ch) {
+        if (ch >= nch) return;
+        const int p0 = ch * AE, na = min(AE, p.P - p0);
+        float4* inf = info + (ch % 3) * AE;
+        int2* kin = kinfo + (ch % 3) * AE;
+        for (int a = tid; a < na; a += NT) {
+            const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
+            const int row = (ch & 1) * AE * W + a * W;
+            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow)));
+            kin[a] = make_int2(w.kmin, w.kabs);
         }
-    }
+    };
+    make_info(0);
+    make_info(1);
     __syncthreads();
 
-    const int nch = (p.P + AE - 1) / AE;
-    HRow<B> pre[E];
+    uint2 q0r[E], q1r[E];
     auto fetch = [&](int ch) {
         const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
+        const int2* kin = kinfo + (ch % 3) * AE;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
+            q0r[q] = make_uint2(0u, 0u);
+            q1r[q] = make_uint2(0u, 0u);
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
-                const int k = kinfo[p0 + a].y + e - a * W;
-                const long off = (long)(p0 + a) * p.D;
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                    float s0 = 0.0f, s1 = 0.0f;
-                    if (b < nb) {
-                        const uint16_t* row = sino + (long)(b0 + b) * plane + off;
-                        if (k >= 0 && k < p.D) s0 = half_bits_to_float(row[k]);
-                        if (k + 1 >= 0 && k + 1 < p.D) s1 = half_bits_to_float(row[k + 1]);
-                    }
-                    pre[q].v[b].x = (_Float16)s0;
-                    pre[q].v[b].y = (_Float16)(s1 - s0);
-                }
+                const int k = kin[a].y + e - a * W;
+                const uint2* row = S + (long)(p0 + a) * p.D;
+                if (k >= 0 && k < p.D) q0r[q] = row[k];
+                if (k + 1 >= 0 && k + 1 < p.D) q1r[q] = row[k + 1];
             }
         }
     };
     auto commit = [&](int ch) {
         const int nae = min(AE, p.P - ch * AE) * W;
-        HRow<B>* buf = rows + (ch & 1) * AE * W;
+        HRow* buf = rows + (ch & 1) * AE * W;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
-            if (e < nae) buf[e] = pre[q];
+            if (e < nae) buf[e] = stage_pairs(q0r[q], q1r[q]);
         }
     };
     fetch(0);
@@ -431,15 +474,16 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
     const char* rb = reinterpret_cast<const char*>(rows);
     for (int ch = 0; ch < nch; ch++) {
         const int p0 = ch * AE, na = min(AE, p.P - p0);
+        const float4* inf = info + (ch % 3) * AE;
         if (ch + 1 < nch) fetch(ch + 1);
         if (wave_active) {
             int a = 0;
             for (; a + U <= na; a += U) {
                 float4 in[U];
-                HRow<B> v[U][PPT];
+                HRow v[U][PPT];
                 h2_t wgt[U][PPT];
 #pragma unroll
-                for (int u = 0; u < U; u++) in[u] = info[p0 + a + u];
+                for (int u = 0; u < U; u++) in[u] = inf[a + u];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
@@ -449,9 +493,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
                         int k;
                         float f;
                         bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
-                        wgt[u][r].x = (_Float16)1.0f;
-                        wgt[u][r].y = (_Float16)f;
-                        v[u][r] = *reinterpret_cast<const HRow<B>*>(row + k * (int)sizeof(HRow<B>));
+                        wgt[u][r] = weight_pair(f);
+                        v[u][r] = *reinterpret_cast<const HRow*>(row + k * (int)sizeof(HRow));
                     }
                 }
 #pragma unroll
@@ -463,7 +506,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
                             acc[r][b] = __builtin_amdgcn_fdot2(wgt[u][r], v[u][r].v[b], acc[r][b], false);
             }
             for (; a < na; a++) {
-                const float4 in = info[p0 + a];
+                const float4 in = inf[a];
                 const float xc = __builtin_fmaf(xp, in.x, in.z);
                 const char* row = rb + __float_as_int(in.w);
 #pragma unroll
@@ -471,16 +514,15 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
                     int k;
                     float f;
                     bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
-                    h2_t wgt;
-                    wgt.x = (_Float16)1.0f;
-                    wgt.y = (_Float16)f;
-                    const HRow<B> v = *reinterpret_cast<const HRow<B>*>(row + k * (int)sizeof(HRow<B>));
+                    const h2_t wgt = weight_pair(f);
+                    const HRow v = *reinterpret_cast<const HRow*>(row + k * (int)sizeof(HRow));
 #pragma unroll
                     for (int b = 0; b < B; b++) acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.v[b], acc[r][b], false);
                 }
             }
         }
         if (ch + 1 < nch) commit(ch + 1);
+        make_info(ch + 2);
         __syncthreads();
     }
     for (int b = 0; b < nb; b++)
@@ -492,6 +534,24 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint1
                     use[r] ? acc[r][b] * p.norm * p.out_scale : p.maskingValue;
         }
 }
+
+// [slices][P][D] FP16 -> the x4 layout (zero-filled past the last slice).
+__global__ void f16_to_x4_kernel(const uint16_t* __restrict__ in, uint2* __restrict__ out, int slices, long plane)
+{
+    const long ngroups = (slices + kF16Group - 1) / kF16Group;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ngroups * plane;
+         i += (long)gridDim.x * blockDim.x) {
+        const long g = i / plane, e = i - g * plane;
+        uint16_t h[kF16Group];
+#pragma unroll
+        for (int b = 0; b < kF16Group; b++) {
+            const long s = g * kF16Group + b;
+            h[b] = s < slices ? in[s * plane + e] : (uint16_t)0;
+        }
+        out[i] = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+    }
+}
+
 // Nearest-neighbour variants (backProjectNearest :305-336, backProjectTex
 // :273-302), one thread per pixel.
 __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restrict__ img,
@@ -637,15 +697,14 @@ static const BpVariant kVariants[] = {
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 struct BpF16Variant {
-    int tx, ty, ppt, b, e, u, amax;
-    void (*fn)(const uint16_t*, float*, const float*, const float*, int, BpParams);
+    int tx, ty, ppt, e, u, amax;
+    void (*fn)(const uint2*, float*, const float*, const float*, int, BpParams);
 };
-#define RISA_BPH(TX, TY, PPT, B, E, U, AMAX) \
-    {TX, TY, PPT, B, E, U, AMAX, bp_linear_f16_kernel<TX, TY, PPT, B, E, U>}
+#define RISA_BPH(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U>}
 static const BpF16Variant kF16Variants[] = {
-    RISA_BPH(32, 16, 2, 4, 4, 2, 32), RISA_BPH(16, 16, 1, 4, 2, 4, 32), RISA_BPH(32, 32, 4, 2, 4, 2, 64),
-    RISA_BPH(32, 16, 2, 2, 4, 4, 64), RISA_BPH(32, 16, 4, 4, 4, 2, 32), RISA_BPH(16, 16, 2, 4, 4, 2, 32),
-    RISA_BPH(32, 8, 1, 4, 2, 4, 32),  RISA_BPH(16, 16, 1, 8, 2, 2, 16),
+    RISA_BPH(32, 16, 2, 4, 2, 64), RISA_BPH(16, 16, 1, 4, 4, 64), RISA_BPH(32, 32, 4, 4, 2, 64),
+    RISA_BPH(16, 16, 2, 4, 2, 64), RISA_BPH(32, 8, 1, 4, 4, 64),  RISA_BPH(16, 32, 2, 4, 2, 64),
+    RISA_BPH(32, 16, 4, 4, 2, 64), RISA_BPH(16, 16, 1, 8, 2, 64),
 };
 #undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
@@ -661,15 +720,18 @@ static int window_entries(int tx, int ty, float scale)
 }
 
 // Angles per chunk and the dynamic LDS bytes of a tile:
-// info[P] (16 B) + 2 buffers x AE x W entries of entry_bytes.
+// 2 row buffers x AE x W entries of entry_bytes + a ring of 3 chunks of
+// per-angle windows (16 B + 8 B per angle).
 constexpr size_t kMaxLds = 64 * 1024;
-static size_t tile_lds(int P, int AE, int W, int entry_bytes) { return (size_t)P * 24 + (size_t)2 * AE * W * entry_bytes; }
-// AE also shrinks so the tile fits the LDS budget (when P leaves room at all).
-static int chunk_angles(int threads, int e, int amax, int W, int P = 0, int entry_bytes = 8)
+static size_t tile_lds(int AE, int W, int entry_bytes)
+{
+    return (size_t)2 * AE * W * entry_bytes + (size_t)3 * AE * 24;
+}
+// AE also shrinks so the tile fits the LDS budget.
+static int chunk_angles(int threads, int e, int amax, int W, int entry_bytes = 8)
 {
     int ae = std::min(amax, e * threads / W);
-    const long room = (long)kMaxLds - (long)P * 24;
-    if (room > 0) ae = std::min<long>(ae, room / (2L * W * entry_bytes));
+    ae = std::min<long>(ae, (long)kMaxLds / (2L * W * entry_bytes + 72));
     return std::max(1, ae);
 }
 
@@ -683,6 +745,7 @@ struct risa_bp_s {
     int variant = 0;
     int variant_f16 = 0;
     bool fuse_mask = false;
+    risa::DeviceBuffer<uint2> x4;   // staging for risa_hip_bp_run_f16's plain layout
 };
 
 using namespace risa;
@@ -735,8 +798,8 @@ static void launch_linear(risa_bp_t h, int variant, const float* sino, float* im
     const BpVariant& v = kVariants[variant];
     BpParams p = make_params(h, v.tx, v.ty);
     const int threads = v.tx * v.ty / v.ppt;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 8);
-    const size_t lds = tile_lds(p.P, p.AE, p.W, sizeof(float2));
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8);
+    const size_t lds = tile_lds(p.AE, p.W, sizeof(float2));
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), slices);
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, p);
@@ -760,25 +823,40 @@ extern "C" int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void*
     });
 }
 
-// Batched FP32 launch (slices of one geometry); used by the batched chain.
-int risa_bp_run_batch_f32(risa_bp_t h, const float* sino, float* img, int slices, hipStream_t s)
+extern "C" int risa_hip_bp_run_batch(risa_bp_t h, const float* sino, int slices, float* img, void* stream)
 {
-    return guarded("risa_bp_run_batch_f32", [&] { launch_linear(h, h->variant, sino, img, slices, s); });
+    return guarded("risa_hip_bp_run_batch", [&] {
+        if (!h || !sino || !img || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (h->g.useTextureMemory || h->g.interpolationType == 1)
+            fail(RISA_ERR_UNSUPPORTED, "batched backprojection is linear interpolation only");
+        if (slices) launch_linear(h, h->variant, sino, img, slices, as_stream(stream));
+    });
 }
 
-static void launch_f16(risa_bp_t h, int variant, const uint16_t* sino, float in_scale, int slices, float* img,
-                       hipStream_t s)
+static void launch_f16x4(risa_bp_t h, int variant, const uint2* sino, float in_scale, int slices, float* img,
+                         hipStream_t s)
 {
     const BpF16Variant& v = kF16Variants[variant];
     BpParams p = make_params(h, v.tx, v.ty);
     p.out_scale = 1.0f / in_scale;
     const int threads = v.tx * v.ty / v.ppt;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 4 * v.b);
-    const size_t lds = tile_lds(p.P, p.AE, p.W, 4 * v.b);
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, (int)sizeof(HRow));
+    const size_t lds = tile_lds(p.AE, p.W, (int)sizeof(HRow));
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
-    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.b));
+    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, kF16Group));
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_f16_kernel");
+}
+
+extern "C" int risa_hip_bp_run_f16x4(risa_bp_t h, const uint16_t* sino_x4, float in_scale, int slices, float* img,
+                                     void* stream)
+{
+    return guarded("risa_hip_bp_run_f16x4", [&] {
+        if (!h || !sino_x4 || !img || slices < 0 || !(in_scale > 0)) fail(RISA_ERR_ARG, "bad argument");
+        if (reinterpret_cast<uintptr_t>(sino_x4) % 8) fail(RISA_ERR_ARG, "x4 sinogram must be 8-byte aligned");
+        if (slices) launch_f16x4(h, h->variant_f16, reinterpret_cast<const uint2*>(sino_x4), in_scale, slices, img,
+                                 as_stream(stream));
+    });
 }
 
 extern "C" int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_scale, int slices, float* img,
@@ -786,7 +864,18 @@ extern "C" int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_s
 {
     return guarded("risa_hip_bp_run_f16", [&] {
         if (!h || !sino || !img || slices < 0 || !(in_scale > 0)) fail(RISA_ERR_ARG, "bad argument");
-        if (slices) launch_f16(h, h->variant_f16, sino, in_scale, slices, img, as_stream(stream));
+        if (!slices) return;
+        DeviceGuard dg(h->device);
+        hipStream_t s = as_stream(stream);
+        const long plane = (long)h->g.numberOfParallelProjections * h->g.numberOfParallelDetectors;
+        const size_t need = (size_t)ceil_div(slices, kF16Group) * plane;
+        if (h->x4.count < need) {
+            RISA_HIP_CHECK(hipStreamSynchronize(s));
+            h->x4.resize(need);
+        }
+        hipLaunchKernelGGL(f16_to_x4_kernel, dim3(1024), dim3(256), 0, s, sino, h->x4.ptr, slices, plane);
+        check_launch("f16_to_x4_kernel");
+        launch_f16x4(h, h->variant_f16, h->x4.ptr, in_scale, slices, img, s);
     });
 }
 
@@ -834,10 +923,10 @@ int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices)
         const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
         const int B = std::max(8, slices);
         DeviceBuffer<float> sino((size_t)P * D * slices);
-        DeviceBuffer<uint16_t> sino16((size_t)P * D * B);
+        DeviceBuffer<uint2> sino16((size_t)P * D * ceil_div(B, kF16Group));
         DeviceBuffer<float> img((size_t)N * N * B);
         RISA_HIP_CHECK(hipMemsetAsync(sino.ptr, 0, sino.count * sizeof(float), s));
-        RISA_HIP_CHECK(hipMemsetAsync(sino16.ptr, 0, sino16.count * sizeof(uint16_t), s));
+        RISA_HIP_CHECK(hipMemsetAsync(sino16.ptr, 0, sino16.count * sizeof(uint2), s));
         hipEvent_t e0, e1;
         RISA_HIP_CHECK(hipEventCreate(&e0));
         RISA_HIP_CHECK(hipEventCreate(&e1));
@@ -866,7 +955,7 @@ int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices)
         for (int v = 0; v < kNumF16Variants; v++) {
             float ms;
             try {
-                ms = time_it([&] { launch_f16(h, v, sino16.ptr, 1.0f, B, img.ptr, s); });
+                ms = time_it([&] { launch_f16x4(h, v, sino16.ptr, 1.0f, B, img.ptr, s); });
             } catch (const Error& e) {
                 if (e.code != RISA_ERR_UNSUPPORTED) throw;
                 continue;
@@ -996,8 +1085,8 @@ static void batched_layout(risa_bp_batched_s* h)
         const int tilesY = ceil_div(j.N, v.ty);
         p.invW = 1.0f / (float)p.W;
         p.out_scale = j.dtype == RISA_F16 ? 1.0f / j.in_scale : 1.0f;
-        p.AE = chunk_angles(threads, v.e, v.amax, p.W, p.P, 8);
-        lds = std::max(lds, tile_lds(p.P, p.AE, p.W, sizeof(float2)));
+        p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8);
+        lds = std::max(lds, tile_lds(p.AE, p.W, sizeof(float2)));
         dj[i].sino = j.sino;
         dj[i].img = j.img;
         dj[i].lut_off = h->lut_off[i];

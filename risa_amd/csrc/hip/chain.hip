@@ -10,8 +10,9 @@
 //        to `lanes` streams (own scratch buffers each) forked from and joined
 //        back into the caller's stream, so several frames are in flight --
 //        the way the reference overlaps frames across its per-stage threads.
-// MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA with FP16
-//        output, FP16-sinogram / FP32-accumulate backprojection.
+// MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA writing
+//        the FP16 sinogram in the 4-slice interleaved layout, FP16-sinogram /
+//        FP32-accumulate backprojection.
 #include "device_common.h"
 
 #include <map>
@@ -98,7 +99,7 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
             } else {
                 h->att_buf.resize(fan * max_frames);
                 h->par_buf.resize(par * max_frames);
-                h->f16_buf.resize(par * max_frames);
+                h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
             // tile shape tuned for the concurrency the chain runs the kernel at
             ok(risa_bp_autotune_slices(h->bp, nullptr, h->lanes));
@@ -140,8 +141,8 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
     }
     ok(risa_hip_attenuation_run_batch(h->att, raw, planes, frames, h->att_buf.ptr, s));
     ok(risa_hip_fan2para_run_batch(h->f2p, h->att_buf.ptr, planes, frames, h->par_buf.ptr, s));
-    ok(risa_hip_filter_run_f16(h->filt, h->par_buf.ptr, h->f16_buf.ptr, frames, kF16SinoScale, s));
-    ok(risa_hip_bp_run_f16(h->bp, h->f16_buf.ptr, kF16SinoScale, frames, img, s));
+    ok(risa_hip_filter_run_f16x4(h->filt, h->par_buf.ptr, h->f16_buf.ptr, frames, kF16SinoScale, s));
+    ok(risa_hip_bp_run_f16x4(h->bp, h->f16_buf.ptr, kF16SinoScale, frames, img, s));
 }
 
 extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img,

@@ -239,78 +239,156 @@ __global__ void reduce_taps_kernel(const float* __restrict__ partial, int nblock
 // Y[M][D] = X[M][D] C[D][D] with C[m][n] = c[(n - m) mod D] (symmetric).  X is
 // read as FP32 and split in the kernel into hi = fp16(sx * X) and
 // lo = fp16(sx * X - hi); C is pre-split on the host the same way
-// (sc * C).  Three MFMAs per k-step accumulate hi*hi + lo*hi + hi*lo in FP32.
-// Tile: 64 x 64 outputs per workgroup of 256 threads (4 waves, each a
-// 32 x 32 block), K step 32.
+// (sc * C).  Three MFMAs per k-step accumulate hi*hi + lo*hi + hi*lo in FP32
+// (v_mfma_f32_32x32x16_f16).
+//
+// Tile: 128 x 128 outputs per workgroup of 512 threads (8 waves, each a
+// 32 x 64 block = two 32 x 32 MFMA blocks sharing the A operand), K step 32
+// staged in LDS as FP16 hi/lo pairs, 16-byte LDS reads for the operands.
+//
+// Output: FP32 rows and/or FP16 (* out_scale) rows, or the "x4" layout of
+// the FP16 backprojector: GEMM row m' = (g*P + a)*4 + b holds slice 4g+b,
+// angle a, so an MFMA lane's four consecutive rows are the four slices of one
+// (angle, detector) and leave as one 8-byte store.
 // ---------------------------------------------------------------------------
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 
-constexpr int GB_M = 64, GB_N = 64, GB_K = 32;
+constexpr int GB_M = 128, GB_N = 128, GB_K = 32, GB_PITCH = GB_K + 8;
 
-__global__ void __launch_bounds__(256) dft_gemm_split_kernel(const float* __restrict__ X, int M, int D,
+struct GemmOut {
+    float* y32;        // [M][D] or null
+    uint16_t* y16;     // [M][D] or null
+    uint2* x4;         // x4 layout or null
+    float out_scale;   // FP16 outputs store value * out_scale
+    int slices, P;     // x4: slices and angles per slice
+};
+
+__device__ __forceinline__ void split8(const float* v, float sx, half8_t& hi, half8_t& lo)
+{
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const float x = v[u] * sx;
+        const _Float16 h = (_Float16)x;
+        hi[u] = h;
+        lo[u] = (_Float16)(x - (float)h);
+    }
+}
+
+__global__ void __launch_bounds__(512) dft_gemm_split_kernel(const float* __restrict__ X, int M, int D,
                                                              const _Float16* __restrict__ Ch,
                                                              const _Float16* __restrict__ Cl, float sx,
-                                                             float inv_scale, float* __restrict__ Y32,
-                                                             uint16_t* __restrict__ Y16, float out_scale)
+                                                             float inv_scale, GemmOut o)
 {
-    // LDS tiles, K-major per row so a lane's 4 consecutive k are contiguous.
-    __shared__ _Float16 Xh[GB_M][GB_K + 4];
-    __shared__ _Float16 Xl[GB_M][GB_K + 4];
-    __shared__ _Float16 Bh[GB_N][GB_K + 4];   // C^T tile: [n][k]
-    __shared__ _Float16 Bl[GB_N][GB_K + 4];
+    __shared__ __attribute__((aligned(16))) _Float16 Ah[GB_M][GB_PITCH];
+    __shared__ __attribute__((aligned(16))) _Float16 Al[GB_M][GB_PITCH];
+    __shared__ __attribute__((aligned(16))) _Float16 Bh[GB_N][GB_PITCH];   // C^T tile [n][k] = C[n][k]
+    __shared__ __attribute__((aligned(16))) _Float16 Bl[GB_N][GB_PITCH];
 
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N;
-    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 64;
+    const bool x4 = o.x4 != nullptr;
+    const int groups = (o.slices + 3) / 4;
+    const int rows = x4 ? groups * o.P * 4 : M;
 
-    float16_t acc = {};
-    for (int k0 = 0; k0 < D; k0 += GB_K) {
-        // stage X tile (64 x 32 floats): 2048 values, 8 per thread
-        for (int e = tid; e < GB_M * GB_K; e += 256) {
-            const int r = e / GB_K, c = e % GB_K;
-            const int gm = m0 + r, gk = k0 + c;
-            float v = (gm < M && gk < D) ? X[(long)gm * D + gk] * sx : 0.0f;
-            const _Float16 h = (_Float16)v;
-            Xh[r][c] = h;
-            Xl[r][c] = (_Float16)(v - (float)h);
+    // this thread's staging slot: row sr, 8 consecutive k from sc8
+    const int sr = tid / 4, sc8 = (tid % 4) * 8;
+    long xrow = -1;
+    {
+        const int m = m0 + sr;
+        if (m < rows) {
+            if (x4) {
+                const int b = m & 3, ga = m >> 2, g = ga / o.P, a = ga - g * o.P, sl = g * 4 + b;
+                if (sl < o.slices) xrow = (long)sl * o.P + a;
+            } else {
+                xrow = m;
+            }
         }
-        // stage C tile as B^T[n][k] = C[k][n] = C[n][k] (symmetric)
-        for (int e = tid; e < GB_N * GB_K; e += 256) {
-            const int r = e / GB_K, c = e % GB_K;
-            const int gn = n0 + r, gk = k0 + c;
-            const bool ok = gn < D && gk < D;
-            Bh[r][c] = ok ? Ch[(long)gn * D + gk] : (_Float16)0.0f;
-            Bl[r][c] = ok ? Cl[(long)gn * D + gk] : (_Float16)0.0f;
+    }
+    const int cn = n0 + sr;
+    const bool vec = (D % 8) == 0;
+
+    float16_t acc0 = {}, acc1 = {};
+    for (int k0 = 0; k0 < D; k0 += GB_K) {
+        const int gk = k0 + sc8;
+        {
+            float v[8];
+            if (xrow >= 0 && vec && gk + 8 <= D) {
+                const float4* src = reinterpret_cast<const float4*>(X + xrow * D + gk);
+                const float4 a = src[0], b = src[1];
+                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+                v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = (xrow >= 0 && gk + u < D) ? X[xrow * D + gk + u] : 0.0f;
+            }
+            half8_t hi, lo;
+            split8(v, sx, hi, lo);
+            *reinterpret_cast<half8_t*>(&Ah[sr][sc8]) = hi;
+            *reinterpret_cast<half8_t*>(&Al[sr][sc8]) = lo;
+        }
+        {
+            half8_t hi, lo;
+            if (cn < D && vec && gk + 8 <= D) {
+                hi = *reinterpret_cast<const half8_t*>(Ch + (long)cn * D + gk);
+                lo = *reinterpret_cast<const half8_t*>(Cl + (long)cn * D + gk);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const bool ok = cn < D && gk + u < D;
+                    hi[u] = ok ? Ch[(long)cn * D + gk + u] : (_Float16)0.0f;
+                    lo[u] = ok ? Cl[(long)cn * D + gk + u] : (_Float16)0.0f;
+                }
+            }
+            *reinterpret_cast<half8_t*>(&Bh[sr][sc8]) = hi;
+            *reinterpret_cast<half8_t*>(&Bl[sr][sc8]) = lo;
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < GB_K; kk += 8) {
-            const int row = lane % 32, kq = kk + (lane / 32) * 4;
-            half4_t ah, al, bh, bl;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                ah[u] = Xh[wm + row][kq + u];
-                al[u] = Xl[wm + row][kq + u];
-                bh[u] = Bh[wn + row][kq + u];
-                bl[u] = Bl[wn + row][kq + u];
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(al, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(ah, bl, acc, 0, 0, 0);
+        for (int kk = 0; kk < GB_K; kk += 16) {
+            const int r = lane % 32, kq = kk + (lane / 32) * 8;
+            const half8_t ah = *reinterpret_cast<const half8_t*>(&Ah[wm + r][kq]);
+            const half8_t al = *reinterpret_cast<const half8_t*>(&Al[wm + r][kq]);
+            const half8_t bh0 = *reinterpret_cast<const half8_t*>(&Bh[wn + r][kq]);
+            const half8_t bl0 = *reinterpret_cast<const half8_t*>(&Bl[wn + r][kq]);
+            const half8_t bh1 = *reinterpret_cast<const half8_t*>(&Bh[wn + 32 + r][kq]);
+            const half8_t bl1 = *reinterpret_cast<const half8_t*>(&Bl[wn + 32 + r][kq]);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, acc1, 0, 0, 0);
         }
         __syncthreads();
     }
     // D layout: lane -> column lane%32; value v -> row 8*(v/4) + 4*(lane/32) + v%4
-    const int col = n0 + wn + (lane % 32);
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int row = m0 + wm + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        if (row < M && col < D) {
-            const float y = acc[v] * inv_scale;
-            if (Y32) Y32[(long)row * D + col] = y;
-            if (Y16) Y16[(long)row * D + col] = float_to_half_bits(y * out_scale);
+    for (int j = 0; j < 2; j++) {
+        const float16_t& acc = j == 0 ? acc0 : acc1;
+        const int col = n0 + wn + 32 * j + (lane % 32);
+        if (col >= D) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int rb = m0 + wm + 8 * q + 4 * (lane / 32);
+            if (x4) {
+                if (rb >= rows) continue;
+                uint32_t h[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) h[b] = float_to_half_bits(acc[4 * q + b] * inv_scale * o.out_scale);
+                o.x4[(long)(rb >> 2) * D + col] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int row = rb + b;
+                    if (row >= M) continue;
+                    const float y = acc[4 * q + b] * inv_scale;
+                    if (o.y32) o.y32[(long)row * D + col] = y;
+                    if (o.y16) o.y16[(long)row * D + col] = float_to_half_bits(y * o.out_scale);
+                }
+            }
         }
     }
 }
@@ -445,17 +523,26 @@ static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* o
     check_launch("fft_filter_kernel");
 }
 
-static void launch_gemm(risa_filter_t h, const float* in, float* out, uint16_t* out16, float out_scale, long rows,
-                        hipStream_t s)
+static void launch_gemm(risa_filter_t h, const float* in, GemmOut o, long rows, hipStream_t s)
 {
     const int D = h->g.numberOfParallelDetectors;
-    if (rows > 0x7fffffffL) fail(RISA_ERR_UNSUPPORTED, "too many rows");
+    if (rows > 0x3fffffffL) fail(RISA_ERR_UNSUPPORTED, "too many rows");
     const float sx = 1024.0f;   // attenuation line integrals stay far below 60 in magnitude
     const float inv = 1.0f / (sx * h->sc);
-    dim3 grid(ceil_div(D, GB_N), ceil_div(rows, GB_M));
-    hipLaunchKernelGGL(dft_gemm_split_kernel, grid, dim3(256), 0, s, in, (int)rows, D, h->Ch.ptr, h->Cl.ptr, sx, inv,
-                       out, out16, out_scale);
+    const long out_rows = o.x4 ? (long)((o.slices + 3) / 4) * o.P * 4 : rows;
+    dim3 grid(ceil_div(D, GB_N), ceil_div(out_rows, GB_M));
+    hipLaunchKernelGGL(dft_gemm_split_kernel, grid, dim3(512), 0, s, in, (int)rows, D, h->Ch.ptr, h->Cl.ptr, sx, inv,
+                       o);
     check_launch("dft_gemm_split_kernel");
+}
+
+static GemmOut gemm_out(float* y32, uint16_t* y16, float out_scale)
+{
+    GemmOut o{};
+    o.y32 = y32;
+    o.y16 = y16;
+    o.out_scale = out_scale;
+    return o;
 }
 
 extern "C" int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, void* stream)
@@ -476,7 +563,7 @@ extern "C" int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, voi
                 RISA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
                 h->tmp.resize(n);
             }
-            launch_gemm(h, sino, h->tmp.ptr, nullptr, 1.0f, rows, as_stream(stream));
+            launch_gemm(h, sino, gemm_out(h->tmp.ptr, nullptr, 1.0f), rows, as_stream(stream));
             RISA_HIP_CHECK(hipMemcpyAsync(sino, h->tmp.ptr, n * sizeof(float), hipMemcpyDeviceToDevice, as_stream(stream)));
         }
     });
@@ -490,7 +577,24 @@ extern "C" int risa_hip_filter_run_f16(risa_filter_t h, const float* sino, uint1
         const long rows = (long)slices * h->g.numberOfParallelProjections;
         if (rows == 0) return;
         if (h->mode == RISA_FILTER_MODE_FFT32) launch_fft(h, sino, nullptr, out, out_scale, rows, as_stream(stream));
-        else launch_gemm(h, sino, nullptr, out, out_scale, rows, as_stream(stream));
+        else launch_gemm(h, sino, gemm_out(nullptr, out, out_scale), rows, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_filter_run_f16x4(risa_filter_t h, const float* sino, uint16_t* out_x4, int slices,
+                                         float out_scale, void* stream)
+{
+    return guarded("risa_hip_filter_run_f16x4", [&] {
+        if (!h || !sino || !out_x4 || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (reinterpret_cast<uintptr_t>(out_x4) % 8) fail(RISA_ERR_ARG, "x4 output must be 8-byte aligned");
+        if (h->mode != RISA_FILTER_MODE_GEMM_SPLIT)
+            fail(RISA_ERR_UNSUPPORTED, "the x4 FP16 output is written by the DFT-GEMM mode");
+        if (slices == 0) return;
+        GemmOut o = gemm_out(nullptr, nullptr, out_scale);
+        o.x4 = reinterpret_cast<uint2*>(out_x4);
+        o.slices = slices;
+        o.P = h->g.numberOfParallelProjections;
+        launch_gemm(h, sino, o, (long)slices * o.P, as_stream(stream));
     });
 }
 
@@ -499,7 +603,7 @@ int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows,
 {
     return guarded("risa_filter_run_oop", [&] {
         if (h->mode == RISA_FILTER_MODE_FFT32) launch_fft(h, in, out, nullptr, 1.0f, rows, s);
-        else launch_gemm(h, in, out, nullptr, 1.0f, rows, s);
+        else launch_gemm(h, in, gemm_out(out, nullptr, 1.0f), rows, s);
     });
 }
 

@@ -123,6 +123,11 @@ class Filter(_Handle):
         check(load().risa_hip_filter_run_f16(self.h, ptr(sino), ptr(out16), slices, out_scale, _stream()))
         return out16
 
+    def f16x4(self, sino, out_x4, slices: int, out_scale: float):
+        _require_cuda(sino, out_x4)
+        check(load().risa_hip_filter_run_f16x4(self.h, ptr(sino), ptr(out_x4), slices, out_scale, _stream()))
+        return out_x4
+
     def adjoint(self, grad, slices: int = 1):
         _require_cuda(grad)
         check(load().risa_hip_filter_adjoint_run(self.h, ptr(grad), slices, _stream()))
@@ -156,9 +161,19 @@ class Backprojection(_Handle):
         check(load().risa_hip_bp_run(self.h, ptr(sino), ptr(img), _stream()))
         return img
 
+    def batch(self, sino, slices: int, img):
+        _require_cuda(sino, img)
+        check(load().risa_hip_bp_run_batch(self.h, ptr(sino), slices, ptr(img), _stream()))
+        return img
+
     def f16(self, sino16, in_scale: float, slices: int, img):
         _require_cuda(sino16, img)
         check(load().risa_hip_bp_run_f16(self.h, ptr(sino16), in_scale, slices, ptr(img), _stream()))
+        return img
+
+    def f16x4(self, sino_x4, in_scale: float, slices: int, img):
+        _require_cuda(sino_x4, img)
+        check(load().risa_hip_bp_run_f16x4(self.h, ptr(sino_x4), in_scale, slices, ptr(img), _stream()))
         return img
 
     def adjoint(self, img, sino):

@@ -265,3 +265,92 @@ def test_chain_mixed_within_2e4(risa, orc, geom_c0, frames_c0, cuda):
         err = rel_rmse(got[f], ref, m)
         print(f"  frame {f}: mixed rel RMSE vs FP64 oracle {err:.3e}", flush=True)
         assert err < 2e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,P,slices", [(512, 1000, 3), (256, 500, 5), (500, 37, 2), (768, 130, 4)])
+def test_filter_gemm_split_matches_fp64(risa, orc, geom_c0, cuda, D, P, slices):
+    """DFT-GEMM on MFMA with the FP16 hi+lo split (FP32 output) against the
+    FP64 circulant; the FP16 and x4 FP16 outputs are the same numbers rounded
+    once, bit for bit, with the x4 layout's padding slices zero."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: GEMM filter D={D} P={P} slices={slices}", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=P, filterType=1)
+    rng = np.random.default_rng(D * 7 + P)
+    x = (rng.standard_normal((slices, P, D)) * 2.0).astype(np.float32)
+    filt = device.Filter(g, mode=1)
+    y = _t(x, cuda)
+    filt(y, slices)
+    _sync()
+    ref = orc.filter_rows_f64(orc.taps_f64(g), x.astype(np.float64))
+    err = rel_rmse(y.cpu().numpy(), ref)
+    print(f"  GEMM FP32 output rel RMSE vs FP64 {err:.2e}", flush=True)
+    assert err < 5e-6
+    scale = 256.0
+    h16 = torch.empty((slices, P, D), dtype=torch.float16, device=cuda)
+    filt.f16(_t(x, cuda), h16, slices, scale)
+    groups = (slices + 3) // 4
+    x4 = torch.full((groups, P, D, 4), float("nan"), dtype=torch.float16, device=cuda)
+    filt.f16x4(_t(x, cuda), x4, slices, scale)
+    _sync()
+    a = h16.cpu().numpy().view(np.uint16)
+    b = x4.cpu().numpy().view(np.uint16)
+    for s in range(groups * 4):
+        lane = b[s // 4, :, :, s % 4]
+        if s < slices:
+            np.testing.assert_array_equal(lane, a[s])
+        else:
+            assert (lane == 0).all()
+    err16 = rel_rmse(a.view(np.float16).astype(np.float64) / scale, ref)
+    print(f"  FP16 output rel RMSE vs FP64 {err16:.2e}", flush=True)
+    assert err16 < 5e-4
+    filt.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("slices", [1, 4, 6])
+def test_backprojection_f16_matches_oracle(risa, orc, geom_c1, cuda, slices):
+    """FP16-sinogram / FP32-accumulate backprojection: the plain-layout and
+    x4-layout entry points give identical images, within the FP16 weight
+    rounding of the oracle on the dequantised sinogram, for every tile
+    variant; padding slices of the last group are not written."""
+    import torch
+    from risa_amd import device, synth
+
+    g = geom_c1.replace(numberOfParallelProjections=360)
+    P, D, N = g.numberOfParallelProjections, g.numberOfParallelDetectors, g.numberOfPixels
+    scale = 256.0
+    sino = np.stack([synth.parallel_sinogram(70 + s, 190.0, P, D) for s in range(slices)])
+    q = (sino * scale).astype(np.float16)
+    deq = q.astype(np.float32) / np.float32(scale)
+    groups = (slices + 3) // 4
+    x4 = np.zeros((groups * 4, P, D), np.float16)
+    x4[:slices] = q
+    x4 = np.ascontiguousarray(x4.reshape(groups, 4, P, D).transpose(0, 2, 3, 1))
+    bp = device.Backprojection(g)
+    m = disk_mask(N)
+    refs = [orc.bp_linear_f32(g, deq[s]) for s in range(slices)]
+    v = 0
+    while True:
+        try:
+            bp.variant = v << 8
+        except Exception:
+            break
+        print(f"case: FP16 BP variant {v} slices {slices}", flush=True)
+        img_a = torch.full((slices + 1, N, N), 7.0, dtype=torch.float32, device=cuda)
+        img_b = torch.full((slices + 1, N, N), 7.0, dtype=torch.float32, device=cuda)
+        bp.f16(_t(q.view(np.uint16), cuda), scale, slices, img_a)
+        bp.f16x4(_t(x4.view(np.uint16), cuda), scale, slices, img_b)
+        _sync()
+        a, b = img_a.cpu().numpy(), img_b.cpu().numpy()
+        np.testing.assert_array_equal(a, b)
+        assert (a[slices] == 7.0).all()
+        worst = max(rel_rmse(a[s], refs[s], m) for s in range(slices))
+        print(f"  variant {v}: worst rel RMSE vs oracle on dequantised input {worst:.2e}", flush=True)
+        assert worst < 2e-4
+        assert all((a[s][~m] == g.maskingValue).all() for s in range(slices))
+        v += 1
+    assert v >= 4
+    bp.close()
