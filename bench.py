@@ -35,6 +35,7 @@ METRIC = "512² slices/sec on 1 GPU (mixed vs plain FP32) at stated RMSE; BP HBM
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_UPDATE = 10          # t: mul+fma; two weights; two accumulate FMAs
+BP_SLICES_IN_FLIGHT = 4       # the plain chain keeps 4 frames in flight
 
 
 def log(*a):
@@ -81,14 +82,18 @@ def kernel_only(args, g, dev):
             bp = device.Backprojection(g)
             bp.autotune()
             if args.kernel == "bp":
-                sino = torch.from_numpy((rng.standard_normal((P, D)) * 0.01).astype(np.float32)).to(dev)
-                img = torch.empty((N, N), dtype=torch.float32, device=dev)
-                fn, per, what = (lambda: bp(sino, img)), 1, f"bp FP32, 1 slice {N}^2 x {P} angles, variant {bp.variant}"
+                S = BP_SLICES_IN_FLIGHT
+                bp.autotune_slices(S)
+                sino = torch.from_numpy((rng.standard_normal((S, P, D)) * 0.01).astype(np.float32)).to(dev)
+                img = torch.empty((S, N, N), dtype=torch.float32, device=dev)
+                fn, per = (lambda: bp.batch(sino, S, img)), S
+                what = f"bp FP32, {S} slices {N}^2 x {P} angles per launch, variant {bp.variant & 0xff}"
             else:
-                B = 16
-                sino = torch.from_numpy((rng.standard_normal((B, P, D)) * 2.0).astype(np.float16)).to(dev)
+                B = 32
+                sino = torch.from_numpy((rng.standard_normal((B // 4, P, D, 4)) * 2.0).astype(np.float16)).to(dev)
                 img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
-                fn, per, what = (lambda: bp.f16(sino, 256.0, B, img)), B, f"bp FP16 batched, {B} slices {N}^2 x {P}"
+                fn, per = (lambda: bp.f16x4(sino, 256.0, B, img)), B
+                what = f"bp FP16 x4 layout, {B} slices {N}^2 x {P} per launch, variant {bp.variant >> 8}"
         else:
             n = 64 << 20
             src = torch.ones(n, dtype=torch.float32, device=dev)

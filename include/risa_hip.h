@@ -244,6 +244,10 @@ int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* st
 int risa_hip_bp_set_variant(risa_bp_t h, int variant);
 int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
 int risa_hip_bp_autotune(risa_bp_t h, void* stream);
+/* The same, timing the FP32 variants with `slices` slices per launch (the
+ * number of frames a caller keeps in flight) and the FP16 variants with
+ * max(8, slices).  (Autotuning has no reference counterpart.) */
+int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream);
 int risa_hip_bp_destroy(risa_bp_t h);
 
 /* ---------------------------------------------------------------- Masking -- */

@@ -494,7 +494,9 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                         float f;
                         bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
                         wgt[u][r] = weight_pair(f);
-                        v[u][r] = *reinterpret_cast<const HRow*>(row + k * (int)sizeof(HRow));
+                        // entries are 16-byte aligned: one ds_read_b128, 8 lanes per LDS cycle
+                        v[u][r] = *static_cast<const HRow*>(
+                            __builtin_assume_aligned(row + k * (int)sizeof(HRow), 16));
                     }
                 }
 #pragma unroll
@@ -515,7 +517,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                     float f;
                     bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
                     const h2_t wgt = weight_pair(f);
-                    const HRow v = *reinterpret_cast<const HRow*>(row + k * (int)sizeof(HRow));
+                    const HRow v = *static_cast<const HRow*>(__builtin_assume_aligned(row + k * (int)sizeof(HRow), 16));
 #pragma unroll
                     for (int b = 0; b < B; b++) acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.v[b], acc[r][b], false);
                 }
@@ -701,10 +703,12 @@ struct BpF16Variant {
     void (*fn)(const uint2*, float*, const float*, const float*, int, BpParams);
 };
 #define RISA_BPH(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U>}
+// 16-pixel-wide tiles: the 16-byte LDS entries of a half-wave row then
+// stay clear of bank conflicts
 static const BpF16Variant kF16Variants[] = {
-    RISA_BPH(32, 16, 2, 4, 2, 64), RISA_BPH(16, 16, 1, 4, 4, 64), RISA_BPH(32, 32, 4, 4, 2, 64),
-    RISA_BPH(16, 16, 2, 4, 2, 64), RISA_BPH(32, 8, 1, 4, 4, 64),  RISA_BPH(16, 32, 2, 4, 2, 64),
-    RISA_BPH(32, 16, 4, 4, 2, 64), RISA_BPH(16, 16, 1, 8, 2, 64),
+    RISA_BPH(16, 16, 1, 4, 4, 64), RISA_BPH(16, 16, 2, 4, 2, 64), RISA_BPH(16, 32, 2, 4, 2, 64),
+    RISA_BPH(16, 8, 1, 4, 4, 64),  RISA_BPH(16, 16, 1, 4, 8, 64), RISA_BPH(16, 32, 4, 4, 2, 64),
+    RISA_BPH(16, 16, 2, 4, 4, 64), RISA_BPH(16, 16, 1, 8, 4, 64),
 };
 #undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
@@ -915,9 +919,10 @@ extern "C" int risa_hip_bp_get_variant(risa_bp_t h, int* variant)
 
 // `slices` FP32 slices per timed launch stand for that many frames in flight
 // (the chain keeps several frames of one geometry in flight at once).
-int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices)
+extern "C" int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream)
 {
-    return guarded("risa_hip_bp_autotune", [&] {
+    const hipStream_t s = as_stream(stream);
+    return guarded("risa_hip_bp_autotune_slices", [&] {
         if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
         DeviceGuard dg(h->device);
         const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
@@ -971,7 +976,7 @@ int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices)
 // the FP16 batched form) with HIP events and keeps the fastest.
 extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
 {
-    return risa_bp_autotune_slices(h, as_stream(stream), 1);
+    return risa_hip_bp_autotune_slices(h, 1, stream);
 }
 
 extern "C" int risa_hip_bp_destroy(risa_bp_t h)

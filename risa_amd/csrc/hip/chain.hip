@@ -20,7 +20,6 @@
 #include <vector>
 
 int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows, hipStream_t s);
-int risa_bp_autotune_slices(risa_bp_t h, hipStream_t s, int slices);
 
 namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
@@ -102,7 +101,7 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
                 h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
             // tile shape tuned for the concurrency the chain runs the kernel at
-            ok(risa_bp_autotune_slices(h->bp, nullptr, h->lanes));
+            ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
             delete h;

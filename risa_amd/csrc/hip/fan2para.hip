@@ -20,8 +20,15 @@
 
 namespace risa {
 
-struct PlaneList2 {
-    int32_t p[64];
+// Frames of one launch grouped by plane: a thread reads its output's table
+// entry once per plane and applies it to every frame of that plane, so the
+// 48 B/output table is read once per plane and launch, not once per frame.
+constexpr int kMaxFramesPerLaunch = 64;
+struct FrameGroups {
+    int32_t plane[kMaxFramesPerLaunch];   // per group
+    int32_t start[kMaxFramesPerLaunch];   // first index into frame[]
+    int32_t count[kMaxFramesPerLaunch];
+    int32_t frame[kMaxFramesPerLaunch];   // frame offsets within the launch
 };
 
 __device__ __forceinline__ float rebin_ray(const float* __restrict__ fan, uint32_t idx, float ft, float fg, int Df,
@@ -53,23 +60,31 @@ __device__ __forceinline__ float rebin_sample(const float* __restrict__ fan, uin
 
 __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fan, const uint4* __restrict__ table,
                                                     float* __restrict__ par, int Df, int Pf, int D, int P,
-                                                    long table_plane_stride, PlaneList2 planes)
+                                                    long table_plane_stride, FrameGroups fg)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int f = blockIdx.z;
-    if (i >= D) return;
+    // a wave covers an 8 x 8 block of outputs (8 detectors x 8 angles): its
+    // rays then fall in a compact patch of the fan sinogram (a row of 64
+    // detectors spreads over ~20 fan rows), so each gather touches few lines
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int i = blockIdx.x * 16 + (w & 1) * 8 + (l & 7);
+    const int j = blockIdx.y * 16 + (w >> 1) * 8 + (l >> 3);
+    const int grp = blockIdx.z;
+    if (i >= D || j >= P) return;
     const long e = (long)j * D + i;
-    const uint4* q = table + (long)planes.p[f] * table_plane_stride + e * 3;
+    const uint4* q = table + (long)fg.plane[grp] * table_plane_stride + e * 3;
     const uint4 idx = q[0];
     const uint4 ftb = q[1];
     const uint4 fgb = q[2];
-    const float* fr = fan + (long)f * Df * Pf;
-    const float zA = rebin_sample(fr, idx.x, idx.y, __uint_as_float(ftb.x), __uint_as_float(ftb.y),
-                                  __uint_as_float(fgb.x), __uint_as_float(fgb.y), Df, Pf);
-    const float zB = rebin_sample(fr, idx.z, idx.w, __uint_as_float(ftb.z), __uint_as_float(ftb.w),
-                                  __uint_as_float(fgb.z), __uint_as_float(fgb.w), Df, Pf);
-    par[(long)f * D * P + e] = 0.5f * zA + 0.5f * zB;
+    const int n = fg.count[grp], first = fg.start[grp];
+    for (int u = 0; u < n; u++) {
+        const int f = fg.frame[first + u];
+        const float* fr = fan + (long)f * Df * Pf;
+        const float zA = rebin_sample(fr, idx.x, idx.y, __uint_as_float(ftb.x), __uint_as_float(ftb.y),
+                                      __uint_as_float(fgb.x), __uint_as_float(fgb.y), Df, Pf);
+        const float zB = rebin_sample(fr, idx.z, idx.w, __uint_as_float(ftb.z), __uint_as_float(ftb.w),
+                                      __uint_as_float(fgb.z), __uint_as_float(fgb.w), Df, Pf);
+        par[(long)f * D * P + e] = 0.5f * zA + 0.5f * zB;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -225,18 +240,29 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
 {
     const int Df = h->g.numberOfFanDetectors, Pf = h->g.numberOfFanProjections;
     const int D = h->g.numberOfParallelDetectors, P = h->g.numberOfParallelProjections;
-    for (int f0 = 0; f0 < frames; f0 += 64) {
-        const int nf = std::min(64, frames - f0);
-        PlaneList2 pl{};
-        for (int f = 0; f < nf; f++) {
-            if (planes[f0 + f] < 0 || planes[f0 + f] >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
-            pl.p[f] = planes[f0 + f];
+    for (int f0 = 0; f0 < frames; f0 += kMaxFramesPerLaunch) {
+        const int nf = std::min(kMaxFramesPerLaunch, frames - f0);
+        FrameGroups fg{};
+        int groups = 0, pos = 0;
+        for (int k = 0; k < h->g.numberOfPlanes; k++) {
+            int cnt = 0;
+            for (int f = 0; f < nf; f++) {
+                const int pl = planes[f0 + f];
+                if (pl < 0 || pl >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
+                if (pl == k) fg.frame[pos + cnt++] = f;
+            }
+            if (cnt) {
+                fg.plane[groups] = k;
+                fg.start[groups] = pos;
+                fg.count[groups] = cnt;
+                groups++;
+                pos += cnt;
+            }
         }
-        const int bx = D >= 256 ? 256 : 64;
-        dim3 grid(ceil_div(D, bx), P, nf);
-        hipLaunchKernelGGL(rebin_kernel, grid, dim3(bx), 0, s, fan + (size_t)f0 * Df * Pf,
+        dim3 grid(ceil_div(D, 16), ceil_div(P, 16), groups);
+        hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, s, fan + (size_t)f0 * Df * Pf,
                            reinterpret_cast<const uint4*>(h->table.ptr), par + (size_t)f0 * D * P, Df, Pf, D, P,
-                           (long)(h->plane_words / 4), pl);
+                           (long)(h->plane_words / 4), fg);
         check_launch("rebin_kernel");
     }
 }

@@ -184,6 +184,9 @@ class Backprojection(_Handle):
     def autotune(self):
         check(load().risa_hip_bp_autotune(self.h, _stream()))
 
+    def autotune_slices(self, slices: int):
+        check(load().risa_hip_bp_autotune_slices(self.h, slices, _stream()))
+
     @property
     def variant(self) -> int:
         v = C.c_int()
