@@ -124,6 +124,8 @@ def main():
                     help="frame-planes in the CPU baseline sample (about 10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--no-stream", action="store_true", help="skip the streamed C++ stage pipeline leg")
+    ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
     ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
@@ -272,6 +274,26 @@ def main():
         log(f"accuracy vs FP64 oracle: {rmse}")
         oc.close()
 
+    # PCIe-inclusive rate: the C++ stage pipeline (H2D -> ... -> D2H) streaming
+    # pinned host frames; reported beside the device-resident value, never as it.
+    streamed = None
+    exe = ROOT / "risa_amd" / "bin" / "risa_pipeline"
+    if exe.exists() and not args.no_stream:
+        import subprocess
+
+        try:
+            r = subprocess.run([str(exe), str(risa_amd.config_path("c1")), "--frames", str(args.stream_frames),
+                                "--unique", "16"], capture_output=True, text=True, timeout=300)
+            if r.returncode == 0:
+                streamed = json.loads(r.stdout.strip().splitlines()[-1])
+                streamed["what"] = ("C++ glados-style stages, frames streamed from pinned host memory "
+                                    "through H2D .. D2H, one frame in flight per stage")
+            else:
+                log(f"stage pipeline failed: {r.stderr[-500:]}")
+        except Exception as exc:  # reported, not fatal
+            log(f"stage pipeline skipped: {exc}")
+        log(f"streamed stage pipeline: {streamed}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -335,6 +357,7 @@ def main():
         "bp_hbm": {"achieved": round(bp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(bp_gbs / HBM_PEAK_GBS, 5), "bytes_per_launch": bp_bytes},
         "bp_f16": bp16,
+        "streamed_pipeline": streamed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -48,6 +48,12 @@ int device_count()
     return n;
 }
 
+int processors_per_device(const std::string& configFile)
+{
+    const int n = config_int_or(configFile, "processorsPerDevice", 1);
+    return n < 1 ? 1 : n;
+}
+
 risa_geom read_geometry(const std::string& configFile)
 {
     risa_geom g{};
@@ -131,7 +137,8 @@ H2D::H2D(const std::string& configFile)
     const int pool = config_int(configFile, "memPoolSize_H2D");
     const std::size_t n = fan_elems(g);
     route_by_index_ = true;
-    start("H2D", device_count(), pool, n, [n](int, input_type& in, output_type& out, hipStream_t s) {
+    start("H2D", device_count(), processors_per_device(configFile), pool, n,
+          [n](int, int, input_type& in, output_type& out, hipStream_t s) {
         if (in.size() < n) throw std::runtime_error("H2D: frame smaller than the fan sinogram");
         hip_check(hipMemcpyAsync(out.data(), in.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, s),
                   "hipMemcpyAsync");
@@ -163,16 +170,17 @@ Attenuation::Attenuation(const std::string& configFile, const std::vector<float>
 void Attenuation::init(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark)
 {
     const int pool = config_int(configFile, "memPoolSize_attenuation");
-    const int devices = device_count();
-    for (int d = 0; d < devices; d++) {
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
         risa_attenuation_t h = nullptr;
-        check(risa_hip_attenuation_create(&g_, ref.data(), dark.data(), d, &h), "Attenuation");
+        check(risa_hip_attenuation_create(&g_, ref.data(), dark.data(), w / lanes, &h), "Attenuation");
         h_.push_back(h);
     }
-    start("Attenuation", devices, pool, fan_elems(g_), [this](int d, input_type& in, output_type& out, hipStream_t s) {
-        check(risa_hip_attenuation_run(h_[(std::size_t)d], in.data(), (int)in.plane(), out.data(), s),
-              "Attenuation::processor");
-    });
+    start("Attenuation", devices, lanes, pool, fan_elems(g_),
+          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_attenuation_run(h_[(std::size_t)w], in.data(), (int)in.plane(), out.data(), s),
+                    "Attenuation::processor");
+          });
 }
 
 Attenuation::~Attenuation()
@@ -186,16 +194,18 @@ Fan2Para::Fan2Para(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_fan2Para");
-    const int devices = device_count();
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    // the rebinning tables are read-only: one handle per device serves its lanes
     for (int d = 0; d < devices; d++) {
         risa_fan2para_t h = nullptr;
         check(risa_hip_fan2para_create(&g_, d, &h), "Fan2Para");
         h_.push_back(h);
     }
-    start("Fan2Para", devices, pool, par_elems(g_), [this](int d, input_type& in, output_type& out, hipStream_t s) {
-        check(risa_hip_fan2para_run(h_[(std::size_t)d], in.data(), (int)in.plane(), out.data(), s),
-              "Fan2Para::processor");
-    });
+    start("Fan2Para", devices, lanes, pool, par_elems(g_),
+          [this, lanes](int w, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_fan2para_run(h_[(std::size_t)(w / lanes)], in.data(), (int)in.plane(), out.data(), s),
+                    "Fan2Para::processor");
+          });
 }
 
 Fan2Para::~Fan2Para()
@@ -208,14 +218,14 @@ Fan2Para::~Fan2Para()
 Filter::Filter(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
-    const int devices = device_count();
-    for (int d = 0; d < devices; d++) {
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
         risa_filter_t h = nullptr;
-        check(risa_hip_filter_create(&g_, d, RISA_FILTER_MODE_FFT32, &h), "Filter");
+        check(risa_hip_filter_create(&g_, w / lanes, RISA_FILTER_MODE_FFT32, &h), "Filter");
         h_.push_back(h);
     }
-    start("Filter", devices, 0, 0, [this](int d, input_type& img, output_type&, hipStream_t s) {
-        check(risa_hip_filter_run(h_[(std::size_t)d], img.data(), 1, s), "Filter::processor");
+    start("Filter", devices, lanes, 0, 0, [this](int w, int, input_type& img, output_type&, hipStream_t s) {
+        check(risa_hip_filter_run(h_[(std::size_t)w], img.data(), 1, s), "Filter::processor");
     });
 }
 
@@ -230,17 +240,18 @@ Backprojection::Backprojection(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_backProjection");
-    const int devices = device_count();
-    for (int d = 0; d < devices; d++) {
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
         risa_bp_t h = nullptr;
-        check(risa_hip_bp_create(&g_, d, &h), "Backprojection");
+        check(risa_hip_bp_create(&g_, w / lanes, &h), "Backprojection");
         h_.push_back(h);
-        glados::hip::ScopedDevice sd(d);
-        check(risa_hip_bp_autotune(h, nullptr), "Backprojection autotune");
+        glados::hip::ScopedDevice sd(w / lanes);
+        // tile shape for the number of frames the device's lanes keep in flight
+        check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
     }
-    start("Backprojection", devices, pool, img_elems(g_),
-          [this](int d, input_type& in, output_type& out, hipStream_t s) {
-              check(risa_hip_bp_run(h_[(std::size_t)d], in.data(), out.data(), s), "Backprojection::processor");
+    start("Backprojection", devices, lanes, pool, img_elems(g_),
+          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_bp_run(h_[(std::size_t)w], in.data(), out.data(), s), "Backprojection::processor");
           });
 }
 
@@ -254,14 +265,14 @@ Backprojection::~Backprojection()
 Masking::Masking(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
-    const int devices = device_count();
-    for (int d = 0; d < devices; d++) {
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
         risa_mask_t h = nullptr;
-        check(risa_hip_mask_create(&g_, d, &h), "Masking");
+        check(risa_hip_mask_create(&g_, w / lanes, &h), "Masking");
         h_.push_back(h);
     }
-    start("Masking", devices, 0, 0, [this](int d, input_type& img, output_type&, hipStream_t s) {
-        check(risa_hip_mask_run(h_[(std::size_t)d], img.data(), s), "Masking::processor");
+    start("Masking", devices, lanes, 0, 0, [this](int w, int, input_type& img, output_type&, hipStream_t s) {
+        check(risa_hip_mask_run(h_[(std::size_t)w], img.data(), s), "Masking::processor");
     });
 }
 
@@ -277,7 +288,8 @@ D2H::D2H(const std::string& configFile)
     const risa_geom g = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_D2H");
     const std::size_t n = img_elems(g);
-    start("D2H", device_count(), pool, n, [n](int, input_type& in, output_type& out, hipStream_t s) {
+    start("D2H", device_count(), processors_per_device(configFile), pool, n,
+          [n](int, int, input_type& in, output_type& out, hipStream_t s) {
         hip_check(hipMemcpyAsync(out.data(), in.data(), n * sizeof(float), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync");
     });

@@ -50,6 +50,10 @@ std::string config_string_or(const std::string& path, const char* key, const std
 // Number of visible devices the stages spread frames over.
 int device_count();
 
+// Processor threads (streams) per device of a stage: the optional key
+// processorsPerDevice, default 1 (the reference runs one per device).
+int processors_per_device(const std::string& configFile);
+
 template <class InT, class OutT>
 class DeviceStage {
 public:
@@ -70,8 +74,8 @@ public:
         if (img.valid()) {
             // frames from the host are dealt round-robin by index (H2D.cu:92-93);
             // device frames stay on their device
-            const int d = (int)((route_by_index_ ? (int)img.index() : img.device()) % (int)workers_.size());
-            workers_[(size_t)d]->queue.push(std::move(img));
+            const int d = (int)((route_by_index_ ? (int)img.index() : img.device()) % (int)queues_.size());
+            queues_[(size_t)d]->push(std::move(img));
             return;
         }
         finish();
@@ -85,28 +89,41 @@ public:
     }
 
 protected:
-    using compute_fn = std::function<void(int dev, input_type& in, output_type& out, hipStream_t s)>;
+    // fn(worker, device, in, out, stream): worker = device * lanes + lane
+    using compute_fn = std::function<void(int worker, int dev, input_type& in, output_type& out, hipStream_t s)>;
 
     DeviceStage() = default;
 
-    // Starts one processor thread per device.  With out_elements > 0 the
+    // Starts `lanes` processor threads per device, each with its own stream
+    // (lanes = 1 is the reference's one processor per device; more keep
+    // several frames of the stage in flight).  With out_elements > 0 the
     // stage allocates: each device gets a pool of pool_size output buffers
     // and fn writes into a fresh one; with 0 the stage works in place and
     // forwards the input Image itself (input_type must equal output_type).
-    void start(const char* name, int devices, int pool_size, std::size_t out_elements, compute_fn fn)
+    void start(const char* name, int devices, int lanes, int pool_size, std::size_t out_elements, compute_fn fn)
     {
         name_ = name;
         fn_ = std::move(fn);
         allocating_ = out_elements > 0;
+        lanes_ = lanes < 1 ? 1 : lanes;
         for (int d = 0; d < devices; d++) {
-            auto w = std::unique_ptr<Worker>(new Worker);
-            w->device = d;
-            glados::hip::ScopedDevice sd(d);
-            hip_check(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            auto q = std::unique_ptr<glados::Queue<input_type>>(new glados::Queue<input_type>);
+            int pool = -1;
             if (allocating_)
-                w->pool = glados::MemoryPool<typename output_type::manager_type>::instance()->registerStage(
+                pool = glados::MemoryPool<typename output_type::manager_type>::instance()->registerStage(
                     (std::size_t)pool_size, out_elements, d);
-            workers_.push_back(std::move(w));
+            for (int l = 0; l < lanes_; l++) {
+                auto w = std::unique_ptr<Worker>(new Worker);
+                w->index = d * lanes_ + l;
+                w->device = d;
+                w->queue = q.get();
+                w->pool = pool;
+                glados::hip::ScopedDevice sd(d);
+                hip_check(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+                workers_.push_back(std::move(w));
+            }
+            queues_.push_back(std::move(q));
+            pools_.push_back(pool);
         }
         for (auto& w : workers_) w->thread = std::thread(&DeviceStage::loop, this, w.get());
     }
@@ -117,20 +134,23 @@ protected:
     void stop()
     {
         if (!stopped_) finish();
-        for (auto& w : workers_) {
+        for (auto& w : workers_)
             if (w->stream) (void)hipStreamDestroy(w->stream);
-            if (allocating_ && w->pool >= 0)
-                glados::MemoryPool<typename output_type::manager_type>::instance()->freeMemory(w->pool);
-        }
+        for (int pool : pools_)
+            if (allocating_ && pool >= 0)
+                glados::MemoryPool<typename output_type::manager_type>::instance()->freeMemory(pool);
         workers_.clear();
+        queues_.clear();
+        pools_.clear();
     }
 
 private:
     struct Worker {
+        int index = 0;
         int device = 0;
         hipStream_t stream = nullptr;
         int pool = -1;
-        glados::Queue<input_type> queue;
+        glados::Queue<input_type>* queue = nullptr;   // shared by the device's lanes
         std::thread thread;
     };
 
@@ -138,7 +158,7 @@ private:
     {
         std::lock_guard<std::mutex> lock(finish_m_);
         if (stopped_) return;
-        for (auto& w : workers_) w->queue.push(input_type());
+        for (auto& w : workers_) w->queue->push(input_type());   // one sentinel per lane
         for (auto& w : workers_)
             if (w->thread.joinable()) w->thread.join();
         stopped_ = true;
@@ -150,11 +170,11 @@ private:
         try {
             glados::hip::ScopedDevice sd(w->device);
             for (;;) {
-                input_type in = w->queue.take();
+                input_type in = w->queue->take();
                 if (!in.valid()) break;
                 if constexpr (std::is_same<input_type, output_type>::value) {
                     if (!allocating_) {
-                        fn_(w->device, in, in, w->stream);
+                        fn_(w->index, w->device, in, in, w->stream);
                         hip_check(hipStreamSynchronize(w->stream), name_);
                         results_.push(std::move(in));
                         continue;
@@ -162,7 +182,7 @@ private:
                 }
                 output_type out =
                     glados::MemoryPool<typename output_type::manager_type>::instance()->requestMemory(w->pool);
-                fn_(w->device, in, out, w->stream);
+                fn_(w->index, w->device, in, out, w->stream);
                 hip_check(hipStreamSynchronize(w->stream), name_);
                 out.setIdx(in.index());
                 out.setPlane(in.plane());
@@ -174,7 +194,7 @@ private:
             if (!error_) error_ = std::current_exception();
             // drain so producers never block on a dead consumer
             for (;;) {
-                input_type in = w->queue.take();
+                input_type in = w->queue->take();
                 if (!in.valid()) break;
             }
         }
@@ -192,11 +212,14 @@ protected:
 private:
     const char* name_ = "stage";
     compute_fn fn_;
+    int lanes_ = 1;
     bool allocating_ = false;
     bool stopped_ = false;
     std::mutex finish_m_, err_m_;
     std::exception_ptr error_;
     std::vector<std::unique_ptr<Worker>> workers_;
+    std::vector<std::unique_ptr<glados::Queue<input_type>>> queues_;   // per device
+    std::vector<int> pools_;                                           // per device
     glados::Queue<output_type> results_;
 };
 
