@@ -36,6 +36,11 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 template <int S>
 __device__ __forceinline__ float2 ci(float2 a) { return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x); }
 
+// Radix-8 DFT in place, natural order: even / odd radix-4 halves recombined
+// with w8^k = e^{S 2 pi i k / 8}.
+template <int S>
+__device__ __forceinline__ void dft8(float2* a);
+
 template <int S>
 __device__ __forceinline__ void butterfly(float2* a, int r)
 {
@@ -73,6 +78,27 @@ __device__ __forceinline__ void butterfly(float2* a, int r)
         a[2] = cadd(m2, n2);
         a[3] = csub(m2, n2);
     }
+}
+
+template <int S>
+__device__ __forceinline__ void dft8(float2* a)
+{
+    const float h = 0.70710678118654752440f;   // sqrt(2)/2
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    butterfly<S>(e, 4);
+    butterfly<S>(o, 4);
+    const float2 o1 = cscale(cadd(o[1], ci<S>(o[1])), h);    // w8^1 o1
+    const float2 o2 = ci<S>(o[2]);                            // w8^2 o2
+    const float2 o3 = cscale(csub(ci<S>(o[3]), o[3]), h);    // w8^3 o3
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
 }
 
 // One Stockham pass over a row held in LDS (src -> dst), threads t0..T-1.
@@ -149,6 +175,87 @@ __global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict
         if (out16) {
             out16[ra * D + n] = float_to_half_bits(v.x * out_scale);
             if (hasb) out16[rb * D + n] = float_to_half_bits(v.y * out_scale);
+        }
+    }
+}
+
+// D = 512 in registers: one complex row (a row pair) per wave, lane L holds
+// elements L + 64c (c = 0..7).  Three radix-8 stages (decimation in
+// frequency, n = a + 8b + 64c, k = k0 + 8k1 + 64k2) with two LDS transposes
+// leave the spectrum in the same lane / register distribution as the input,
+// so H multiplies in place and the inverse runs the same stages with
+// conjugate twiddles.  tw[n] = e^{-2 pi i n / 512}.
+template <int S>
+__device__ __forceinline__ void fft512_pass(float2 (&v)[8], float2* buf, const float2 (&w1)[8],
+                                            const float2 (&w2)[8], int L)
+{
+    // stage 1 (lane t = a + 8b): DFT over c, times w^{t k0}
+    dft8<S>(v);
+#pragma unroll
+    for (int k = 1; k < 8; k++) v[k] = cmul(v[k], S < 0 ? w1[k] : make_float2(w1[k].x, -w1[k].y));
+#pragma unroll
+    for (int k = 0; k < 8; k++) buf[k * 65 + L] = v[k];
+    __syncthreads();
+    {
+        // stage 2 (lane a + 8 k0): DFT over b, times w64^{a k1}
+        const int a = L & 7, k0 = L >> 3;
+#pragma unroll
+        for (int b = 0; b < 8; b++) v[b] = buf[k0 * 65 + a + 8 * b];
+        dft8<S>(v);
+#pragma unroll
+        for (int k = 1; k < 8; k++) v[k] = cmul(v[k], S < 0 ? w2[k] : make_float2(w2[k].x, -w2[k].y));
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; k++) buf[(k0 * 8 + k) * 9 + a] = v[k];
+    }
+    __syncthreads();
+    {
+        // stage 3 (lane k0 + 8 k1): DFT over a -> X[L + 64 k2]
+        const int k0 = L & 7, k1 = L >> 3;
+#pragma unroll
+        for (int a = 0; a < 8; a++) v[a] = buf[(k0 * 8 + k1) * 9 + a];
+        dft8<S>(v);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) fft512_filter_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                           uint16_t* __restrict__ out16, float out_scale,
+                                                           const float2* __restrict__ tw,
+                                                           const float* __restrict__ H, long rows)
+{
+    __shared__ float2 buf[576];
+    const int L = threadIdx.x;
+    const long ra = (long)blockIdx.x * 2, rb = ra + 1;
+    const bool hasb = rb < rows;
+    const float* xa = in + ra * 512;
+    const float* xb = in + rb * 512;
+    float2 v[8], w1[8], w2[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = make_float2(xa[L + 64 * c], hasb ? xb[L + 64 * c] : 0.0f);
+    const int a = L & 7;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        w1[k] = tw[L * k];              // w^{t k0}, t k0 < 512
+        w2[k] = tw[(8 * a * k) & 511];  // w64^{a k1}
+    }
+    fft512_pass<-1>(v, buf, w1, w2, L);
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = cscale(v[c], H[L + 64 * c]);
+    // the inverse runs in the distribution the forward left (lane L holds
+    // bins L + 64c), which is the forward's input distribution: same stages,
+    // same twiddle registers, conjugated
+    fft512_pass<+1>(v, buf, w1, w2, L);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const long n = L + 64 * c;
+        if (out) {
+            out[ra * 512 + n] = v[c].x;
+            if (hasb) out[rb * 512 + n] = v[c].y;
+        }
+        if (out16) {
+            out16[ra * 512 + n] = float_to_half_bits(v[c].x * out_scale);
+            if (hasb) out16[rb * 512 + n] = float_to_half_bits(v[c].y * out_scale);
         }
     }
 }
@@ -516,6 +623,12 @@ static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* o
 {
     const int D = h->g.numberOfParallelDetectors;
     const long pairs = (rows + 1) / 2;
+    if (D == 512) {
+        hipLaunchKernelGGL(fft512_filter_kernel, dim3((unsigned)pairs), dim3(64), 0, s, in, out, out16, out_scale,
+                           h->tw.ptr, h->H.ptr, rows);
+        check_launch("fft512_filter_kernel");
+        return;
+    }
     const int grid = ceil_div(pairs, 2);
     const size_t lds = (size_t)2 * 2 * D * sizeof(float2);
     hipLaunchKernelGGL(fft_filter_kernel, dim3(grid), dim3(256), lds, s, in, out, out16, out_scale, h->tw.ptr,
