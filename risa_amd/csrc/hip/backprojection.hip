@@ -369,7 +369,14 @@ __device__ __forceinline__ HRow stage_pairs(uint2 q0, uint2 q1)
     return r;
 }
 
-template <int TX, int TY, int PPT, int E, int U>
+template <int G>
+struct HRowG {
+    HRow h[G];
+};
+
+// G groups of 4 slices per workgroup: the per-pixel t, index and weight are
+// shared by 4G slices.
+template <int TX, int TY, int PPT, int E, int U, int G>
 __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2* __restrict__ sino,
                                                                      float* __restrict__ img,
                                                                      const float* __restrict__ cosT,
@@ -378,20 +385,24 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
 {
     constexpr int NT = TX * TY / PPT;
     constexpr int RS = TY / PPT;
-    constexpr int B = kF16Group;
+    constexpr int B = kF16Group * G;
+    using Entry = HRowG<G>;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int W = p.W, AE = p.AE;
-    HRow* rows = reinterpret_cast<HRow*>(smem);
-    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(HRow));
+    Entry* rows = reinterpret_cast<Entry*>(smem);
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(Entry));
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
-    const int g = blockIdx.y;
-    const int b0 = g * B;
+    const int g0 = blockIdx.y * G;
+    const int ngroups = (slices + kF16Group - 1) / kF16Group;
+    const int ng = min(G, ngroups - g0);
+    const int b0 = g0 * kF16Group;
     const int nb = min(B, slices - b0);
-    const uint2* S = sino + (long)g * p.P * p.D;
+    const uint2* S = sino + (long)g0 * p.P * p.D;
+    const long gstride = (long)p.P * p.D;
 
     const int x = x0 + tx;
     const float xp = ((float)x - p.center) * p.scale;
@@ -433,7 +444,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
             const int row = (ch & 1) * AE * W + a * W;
-            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow)));
+            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(Entry)));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
     };
@@ -441,31 +452,43 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     make_info(1);
     __syncthreads();
 
-    uint2 q0r[E], q1r[E];
+    uint2 q0r[E][G], q1r[E][G];
     auto fetch = [&](int ch) {
         const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
         const int2* kin = kinfo + (ch % 3) * AE;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
-            q0r[q] = make_uint2(0u, 0u);
-            q1r[q] = make_uint2(0u, 0u);
+#pragma unroll
+            for (int gi = 0; gi < G; gi++) {
+                q0r[q][gi] = make_uint2(0u, 0u);
+                q1r[q][gi] = make_uint2(0u, 0u);
+            }
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
                 const int k = kin[a].y + e - a * W;
-                const uint2* row = S + (long)(p0 + a) * p.D;
-                if (k >= 0 && k < p.D) q0r[q] = row[k];
-                if (k + 1 >= 0 && k + 1 < p.D) q1r[q] = row[k + 1];
+#pragma unroll
+                for (int gi = 0; gi < G; gi++) {
+                    if (gi >= ng) break;
+                    const uint2* row = S + gi * gstride + (long)(p0 + a) * p.D;
+                    if (k >= 0 && k < p.D) q0r[q][gi] = row[k];
+                    if (k + 1 >= 0 && k + 1 < p.D) q1r[q][gi] = row[k + 1];
+                }
             }
         }
     };
     auto commit = [&](int ch) {
         const int nae = min(AE, p.P - ch * AE) * W;
-        HRow* buf = rows + (ch & 1) * AE * W;
+        Entry* buf = rows + (ch & 1) * AE * W;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
-            if (e < nae) buf[e] = stage_pairs(q0r[q], q1r[q]);
+            if (e < nae) {
+                Entry en;
+#pragma unroll
+                for (int gi = 0; gi < G; gi++) en.h[gi] = stage_pairs(q0r[q][gi], q1r[q][gi]);
+                buf[e] = en;
+            }
         }
     };
     fetch(0);
@@ -480,7 +503,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
             int a = 0;
             for (; a + U <= na; a += U) {
                 float4 in[U];
-                HRow v[U][PPT];
+                Entry v[U][PPT];
                 h2_t wgt[U][PPT];
 #pragma unroll
                 for (int u = 0; u < U; u++) in[u] = inf[a + u];
@@ -495,8 +518,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                         bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
                         wgt[u][r] = weight_pair(f);
                         // entries are 16-byte aligned: one ds_read_b128, 8 lanes per LDS cycle
-                        v[u][r] = *static_cast<const HRow*>(
-                            __builtin_assume_aligned(row + k * (int)sizeof(HRow), 16));
+                        v[u][r] = *static_cast<const Entry*>(
+                            __builtin_assume_aligned(row + k * (int)sizeof(Entry), 16));
                     }
                 }
 #pragma unroll
@@ -505,7 +528,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                     for (int r = 0; r < PPT; r++)
 #pragma unroll
                         for (int b = 0; b < B; b++)
-                            acc[r][b] = __builtin_amdgcn_fdot2(wgt[u][r], v[u][r].v[b], acc[r][b], false);
+                            acc[r][b] = __builtin_amdgcn_fdot2(wgt[u][r], v[u][r].h[b / 4].v[b % 4], acc[r][b],
+                                                               false);
             }
             for (; a < na; a++) {
                 const float4 in = inf[a];
@@ -517,9 +541,10 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                     float f;
                     bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
                     const h2_t wgt = weight_pair(f);
-                    const HRow v = *static_cast<const HRow*>(__builtin_assume_aligned(row + k * (int)sizeof(HRow), 16));
+                    const Entry v = *static_cast<const Entry*>(__builtin_assume_aligned(row + k * (int)sizeof(Entry), 16));
 #pragma unroll
-                    for (int b = 0; b < B; b++) acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.v[b], acc[r][b], false);
+                    for (int b = 0; b < B; b++)
+                        acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.h[b / 4].v[b % 4], acc[r][b], false);
                 }
             }
         }
@@ -699,16 +724,18 @@ static const BpVariant kVariants[] = {
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 struct BpF16Variant {
-    int tx, ty, ppt, e, u, amax;
+    int tx, ty, ppt, e, u, g, amax;
     void (*fn)(const uint2*, float*, const float*, const float*, int, BpParams);
 };
-#define RISA_BPH(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U>}
+#define RISA_BPH(TX, TY, PPT, E, U, G, AMAX) \
+    {TX, TY, PPT, E, U, G, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U, G>}
 // 16-pixel-wide tiles: the 16-byte LDS entries of a half-wave row then
-// stay clear of bank conflicts
+// stay clear of bank conflicts; G = 2 shares each pixel's weight over 8 slices
 static const BpF16Variant kF16Variants[] = {
-    RISA_BPH(16, 16, 1, 4, 4, 64), RISA_BPH(16, 16, 2, 4, 2, 64), RISA_BPH(16, 32, 2, 4, 2, 64),
-    RISA_BPH(16, 8, 1, 4, 4, 64),  RISA_BPH(16, 16, 1, 4, 8, 64), RISA_BPH(16, 32, 4, 4, 2, 64),
-    RISA_BPH(16, 16, 2, 4, 4, 64), RISA_BPH(16, 16, 1, 8, 4, 64),
+    RISA_BPH(16, 16, 1, 4, 4, 1, 64), RISA_BPH(16, 16, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 2, 4, 2, 1, 64),
+    RISA_BPH(16, 8, 1, 4, 4, 1, 64),  RISA_BPH(16, 32, 4, 4, 2, 1, 64), RISA_BPH(16, 16, 1, 4, 4, 2, 64),
+    RISA_BPH(16, 32, 2, 4, 2, 2, 64), RISA_BPH(16, 8, 1, 4, 4, 2, 64),  RISA_BPH(16, 16, 2, 4, 2, 2, 64),
+    RISA_BPH(16, 16, 1, 4, 2, 2, 64),
 };
 #undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
@@ -844,10 +871,11 @@ static void launch_f16x4(risa_bp_t h, int variant, const uint2* sino, float in_s
     BpParams p = make_params(h, v.tx, v.ty);
     p.out_scale = 1.0f / in_scale;
     const int threads = v.tx * v.ty / v.ppt;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, (int)sizeof(HRow));
-    const size_t lds = tile_lds(p.AE, p.W, (int)sizeof(HRow));
+    const int entry = (int)sizeof(HRow) * v.g;
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, entry);
+    const size_t lds = tile_lds(p.AE, p.W, entry);
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
-    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, kF16Group));
+    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(ceil_div(slices, kF16Group), v.g));
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_f16_kernel");
 }
