@@ -47,42 +47,95 @@ __device__ __forceinline__ float rebin_ray(const float* __restrict__ fan, uint32
     return V1 + fg * (V2 - V1);
 }
 
-// Flag-weighted mean of the two rays of one internal sample (:153-158).
-__device__ __forceinline__ float rebin_sample(const float* __restrict__ fan, uint32_t i1, uint32_t i2, float ft1,
-                                              float ft2, float fg1, float fg2, int Df, int Pf)
+// Fan patches in LDS: an output block of 16 x 16 (detector x angle) draws each
+// of its four ray slots from a small rectangle of the fan sinogram (median
+// ~15 x 15 samples at C1).  The rectangles, built on the host from the integer
+// table, are staged in LDS once per frame and the 16 bilinear taps of every
+// output are read from there; a slot whose rectangle wraps around the
+// theta/gamma seam or exceeds kPatchMax samples (a few per cent of blocks,
+// e.g. the alpha_0 = 0 row) gathers from global memory instead.
+constexpr int kRebinBlock = 16;
+constexpr int kPatchMax = 512;
+
+// Ray of one slot from its LDS patch (no wrap inside a patch).
+__device__ __forceinline__ float patch_ray(const float* __restrict__ pt, int4 box, uint32_t idx, float ft, float fg)
 {
-    const bool r1 = (i1 >> 31) != 0, r2 = (i2 >> 31) != 0;
-    const float z1 = r1 ? rebin_ray(fan, i1, ft1, fg1, Df, Pf) : 0.0f;
-    const float z2 = r2 ? rebin_ray(fan, i2, ft2, fg2, Df, Pf) : 0.0f;
+    const int tb = (int)(idx & 0xFFFFu) - box.x;
+    const int gb = (int)((idx >> 16) & 0x7FFFu) - box.y;
+    const int a = tb * box.w + gb;
+    const float W1 = pt[a], W2 = pt[a + 1], W3 = pt[a + box.w], W4 = pt[a + box.w + 1];
+    const float V1 = W1 + ft * (W3 - W1);
+    const float V2 = W2 + ft * (W4 - W2);
+    return V1 + fg * (V2 - V1);
+}
+
+__device__ __forceinline__ float slot_ray(const float* __restrict__ fr, const float* __restrict__ pt, int4 box,
+                                          uint32_t idx, float ft, float fg, int Df, int Pf)
+{
+    return box.z > 0 ? patch_ray(pt, box, idx, ft, fg) : rebin_ray(fr, idx, ft, fg, Df, Pf);
+}
+
+// Flag-weighted mean of the two rays of one internal sample (:153-158).
+__device__ __forceinline__ float mix_rays(bool r1, bool r2, float z1, float z2)
+{
     if (r1 && r2) return 0.5f * z1 + 0.5f * z2;
-    return r1 ? z1 : z2;
+    return r1 ? z1 : (r2 ? z2 : 0.0f);
 }
 
 __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fan, const uint4* __restrict__ table,
-                                                    float* __restrict__ par, int Df, int Pf, int D, int P,
-                                                    long table_plane_stride, FrameGroups fg)
+                                                    const int4* __restrict__ boxes, float* __restrict__ par, int Df,
+                                                    int Pf, int D, int P, long table_plane_stride,
+                                                    long box_plane_stride, FrameGroups fg)
 {
-    // a wave covers an 8 x 8 block of outputs (8 detectors x 8 angles): its
-    // rays then fall in a compact patch of the fan sinogram (a row of 64
-    // detectors spreads over ~20 fan rows), so each gather touches few lines
+    __shared__ float patch[4 * kPatchMax];
+    // a wave covers an 8 x 8 sub-block, so global fallback gathers stay local too
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int i = blockIdx.x * 16 + (w & 1) * 8 + (l & 7);
-    const int j = blockIdx.y * 16 + (w >> 1) * 8 + (l >> 3);
+    const int i = blockIdx.x * kRebinBlock + (w & 1) * 8 + (l & 7);
+    const int j = blockIdx.y * kRebinBlock + (w >> 1) * 8 + (l >> 3);
     const int grp = blockIdx.z;
-    if (i >= D || j >= P) return;
+    const bool inside = i < D && j < P;
+    const int plane = fg.plane[grp];
+    const int4* bx = boxes + plane * box_plane_stride + ((long)blockIdx.y * gridDim.x + blockIdx.x) * 4;
+    int4 box[4];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) box[sl] = bx[sl];
     const long e = (long)j * D + i;
-    const uint4* q = table + (long)fg.plane[grp] * table_plane_stride + e * 3;
-    const uint4 idx = q[0];
-    const uint4 ftb = q[1];
-    const uint4 fgb = q[2];
+    uint4 idx = make_uint4(0u, 0u, 0u, 0u), ftb = idx, fgb = idx;
+    if (inside) {
+        const uint4* q = table + plane * table_plane_stride + e * 3;
+        idx = q[0];
+        ftb = q[1];
+        fgb = q[2];
+    }
+    const uint32_t id[4] = {idx.x, idx.y, idx.z, idx.w};
+    const float ft[4] = {__uint_as_float(ftb.x), __uint_as_float(ftb.y), __uint_as_float(ftb.z),
+                         __uint_as_float(ftb.w)};
+    const float fgm[4] = {__uint_as_float(fgb.x), __uint_as_float(fgb.y), __uint_as_float(fgb.z),
+                          __uint_as_float(fgb.w)};
     const int n = fg.count[grp], first = fg.start[grp];
     for (int u = 0; u < n; u++) {
         const int f = fg.frame[first + u];
         const float* fr = fan + (long)f * Df * Pf;
-        const float zA = rebin_sample(fr, idx.x, idx.y, __uint_as_float(ftb.x), __uint_as_float(ftb.y),
-                                      __uint_as_float(fgb.x), __uint_as_float(fgb.y), Df, Pf);
-        const float zB = rebin_sample(fr, idx.z, idx.w, __uint_as_float(ftb.z), __uint_as_float(ftb.w),
-                                      __uint_as_float(fgb.z), __uint_as_float(fgb.w), Df, Pf);
+        if (u > 0) __syncthreads();   // the previous frame's patches are consumed
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++) {
+            const int4 b = box[sl];
+            if (b.z <= 0) continue;
+            const int cnt = b.z * b.w;
+            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                const int r = t / b.w, c = t - r * b.w;
+                patch[sl * kPatchMax + t] = fr[(long)(b.x + r) * Df + b.y + c];
+            }
+        }
+        __syncthreads();
+        if (!inside) continue;
+        float z[4];
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++)
+            z[sl] = (id[sl] >> 31) ? slot_ray(fr, patch + sl * kPatchMax, box[sl], id[sl], ft[sl], fgm[sl], Df, Pf)
+                                   : 0.0f;
+        const float zA = mix_rays((id[0] >> 31) != 0, (id[1] >> 31) != 0, z[0], z[1]);
+        const float zB = mix_rays((id[2] >> 31) != 0, (id[3] >> 31) != 0, z[2], z[3]);
         par[(long)f * D * P + e] = 0.5f * zA + 0.5f * zB;
     }
 }
@@ -179,9 +232,48 @@ struct risa_fan2para_s {
     risa_geom g{};
     risa::DeviceBuffer<uint32_t> table;   // [planes][P][D][12]
     size_t plane_words = 0;
+    risa::DeviceBuffer<int4> boxes;       // [planes][block rows][block cols][4 slots]
+    long box_plane = 0;
 };
 
 using namespace risa;
+
+// Per 16 x 16 output block and ray slot, the fan rectangle its rays read:
+// {theta0, gamma0, rows, cols}; rows = 0: no ray of the slot in the block;
+// rows = -1: gather from global memory (the rectangle wraps a seam or is
+// larger than kPatchMax).
+static std::vector<int4> patch_boxes(const PackedRebin& pk, int P, int D, int Pf, int Df)
+{
+    const int bxn = ceil_div(D, kRebinBlock), byn = ceil_div(P, kRebinBlock);
+    std::vector<int4> out((size_t)bxn * byn * 4);
+    for (int by = 0; by < byn; by++)
+        for (int bxi = 0; bxi < bxn; bxi++)
+            for (int sl = 0; sl < 4; sl++) {
+                int tlo = 1 << 30, thi = -1, glo = 1 << 30, ghi = -1;
+                bool wrap = false, any = false;
+                for (int j = by * kRebinBlock; j < std::min(P, (by + 1) * kRebinBlock); j++)
+                    for (int i = bxi * kRebinBlock; i < std::min(D, (bxi + 1) * kRebinBlock); i++) {
+                        const uint32_t idx = pk.q[((size_t)j * D + i) * 12 + sl];
+                        if (!(idx >> 31)) continue;
+                        any = true;
+                        const int tb = (int)(idx & 0xFFFFu), gb = (int)((idx >> 16) & 0x7FFFu);
+                        const int ta = tb + 1 == Pf ? 0 : tb + 1, ga = gb + 1 == Df ? 0 : gb + 1;
+                        if (ta < tb || ga < gb) wrap = true;
+                        tlo = std::min(tlo, tb);
+                        thi = std::max(thi, ta);
+                        glo = std::min(glo, gb);
+                        ghi = std::max(ghi, ga);
+                    }
+                int4 b = make_int4(0, 0, 0, 0);
+                if (any) {
+                    const long rows = thi - tlo + 1, cols = ghi - glo + 1;
+                    if (wrap || rows * cols > kPatchMax) b = make_int4(0, 0, -1, 0);
+                    else b = make_int4(tlo, glo, (int)rows, (int)cols);
+                }
+                out[((size_t)by * bxn + bxi) * 4 + sl] = b;
+            }
+    return out;
+}
 
 extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int weights, risa_fan2para_t* out)
 {
@@ -203,6 +295,13 @@ extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int w
                 const PackedRebin pk = pack_rebin(*g, t, weights == RISA_REBIN_WEIGHTS_FP64);
                 uint32_t* dst = h->table.ptr + k * h->plane_words;
                 RISA_HIP_CHECK(hipMemcpy(dst, pk.q.data(), pk.q.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+                const std::vector<int4> bx = patch_boxes(pk, P, D, g->numberOfFanProjections, g->numberOfFanDetectors);
+                if (k == 0) {
+                    h->box_plane = (long)bx.size();
+                    h->boxes.resize(bx.size() * g->numberOfPlanes);
+                }
+                RISA_HIP_CHECK(hipMemcpy(h->boxes.ptr + k * h->box_plane, bx.data(), bx.size() * sizeof(int4),
+                                         hipMemcpyHostToDevice));
                 if (weights == RISA_REBIN_WEIGHTS_FP64) {
                     RebinGeo rg{};
                     rg.P = P; rg.P2 = 2 * P; rg.D = D;
@@ -259,10 +358,10 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
                 pos += cnt;
             }
         }
-        dim3 grid(ceil_div(D, 16), ceil_div(P, 16), groups);
+        dim3 grid(ceil_div(D, kRebinBlock), ceil_div(P, kRebinBlock), groups);
         hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, s, fan + (size_t)f0 * Df * Pf,
-                           reinterpret_cast<const uint4*>(h->table.ptr), par + (size_t)f0 * D * P, Df, Pf, D, P,
-                           (long)(h->plane_words / 4), fg);
+                           reinterpret_cast<const uint4*>(h->table.ptr), h->boxes.ptr, par + (size_t)f0 * D * P,
+                           Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
         check_launch("rebin_kernel");
     }
 }
