@@ -287,7 +287,7 @@ def main():
             if r.returncode == 0:
                 streamed = json.loads(r.stdout.strip().splitlines()[-1])
                 streamed["what"] = ("C++ glados-style stages, frames streamed from pinned host memory "
-                                    "through H2D .. D2H, one frame in flight per stage")
+                                    "through H2D .. D2H, processorsPerDevice streams per stage")
             else:
                 log(f"stage pipeline failed: {r.stderr[-500:]}")
         except Exception as exc:  # reported, not fatal
