@@ -90,6 +90,7 @@ def kernel_only(args, g, dev):
                 what = f"bp FP32, {S} slices {N}^2 x {P} angles per launch, variant {bp.variant & 0xff}"
             else:
                 B = 32
+                bp.autotune_f16(B)
                 sino = torch.from_numpy((rng.standard_normal((B // 4, P, D, 4)) * 2.0).astype(np.float16)).to(dev)
                 img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
                 fn, per = (lambda: bp.f16x4(sino, 256.0, B, img)), B
@@ -235,6 +236,7 @@ def main():
     bp16 = None
     try:
         B = 32
+        bp.autotune_f16(B)
         s16 = torch.from_numpy((rng.standard_normal((B // 4, P, D, 4)) * 2.0).astype(np.float16)).to(dev)
         im16 = torch.empty((B, N, N), dtype=torch.float32, device=dev)
         with torch.cuda.stream(stream):
@@ -308,9 +310,18 @@ def main():
         t = time.perf_counter()
         oc.f32(sample_raw, sample_planes, threads=threads)
         dt = time.perf_counter() - t
+        cpu_model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": n / dt, "unit": "slices/s", "cores": threads, "kind": "port",
                "sample": f"{n} C1 frame-planes through the faithful-FP32 oracle chain "
-                         f"(own mixed-radix CPU FFT), OpenMP over frames, {dt:.1f}s wall"}
+                         f"(own mixed-radix CPU FFT), OpenMP over frames, {dt:.1f}s wall",
+               "host": f"{cpu_model}, nproc {os.cpu_count()}"}
         log(f"cpu baseline: {cpu}")
         oc.close()
 

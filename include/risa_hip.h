@@ -240,14 +240,20 @@ int risa_hip_bp_run_f16x4(risa_bp_t h, const uint16_t* sino_x4, float in_scale, 
 /* Adjoint of risa_hip_bp_run (linear): the ray-driven projector that is its
  * exact transpose (mask included when fused).  img [N][N] -> sino [P][D]. */
 int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream);
-/* Kernel variant: -1 = autotune on first run (default), else an index. */
+/* Kernel (tile-shape) variants: bits 0-7 the FP32 variant, bits 8-15 the
+ * FP16 one; both start at 0 and are chosen by the autotune calls below. */
 int risa_hip_bp_set_variant(risa_bp_t h, int variant);
 int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
+/* Times every variant that fits the geometry and keeps the fastest: FP32
+ * variants at 1 slice per launch, FP16 variants at 8. */
 int risa_hip_bp_autotune(risa_bp_t h, void* stream);
 /* The same, timing the FP32 variants with `slices` slices per launch (the
  * number of frames a caller keeps in flight) and the FP16 variants with
  * max(8, slices).  (Autotuning has no reference counterpart.) */
 int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream);
+/* FP16 variants only, timed at `slices` slices per risa_hip_bp_run_f16x4
+ * launch (the batch the caller runs). */
+int risa_hip_bp_autotune_f16(risa_bp_t h, int slices, void* stream);
 int risa_hip_bp_destroy(risa_bp_t h);
 
 /* ---------------------------------------------------------------- Masking -- */

@@ -387,10 +387,14 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     constexpr int RS = TY / PPT;
     constexpr int B = kF16Group * G;
     using Entry = HRowG<G>;
+    // one LDS plane per group of 4 slices, each two chunks of at most E*NT
+    // 16-byte entries (AE*W <= E*NT): a group's reads stay 16-byte strided
+    // (no bank conflicts) and the plane offset is a ds_read immediate
+    constexpr int PLANE = 2 * E * NT;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int W = p.W, AE = p.AE;
-    Entry* rows = reinterpret_cast<Entry*>(smem);
-    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(Entry));
+    HRow* rows = reinterpret_cast<HRow*>(smem);
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)G * PLANE * sizeof(HRow));
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int tid = threadIdx.x;
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
             const int row = (ch & 1) * AE * W + a * W;
-            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(Entry)));
+            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow)));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
     };
@@ -479,15 +483,13 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     };
     auto commit = [&](int ch) {
         const int nae = min(AE, p.P - ch * AE) * W;
-        Entry* buf = rows + (ch & 1) * AE * W;
+        HRow* buf = rows + (ch & 1) * AE * W;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
             if (e < nae) {
-                Entry en;
 #pragma unroll
-                for (int gi = 0; gi < G; gi++) en.h[gi] = stage_pairs(q0r[q][gi], q1r[q][gi]);
-                buf[e] = en;
+                for (int gi = 0; gi < G; gi++) buf[gi * PLANE + e] = stage_pairs(q0r[q][gi], q1r[q][gi]);
             }
         }
     };
@@ -517,9 +519,12 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                         float f;
                         bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
                         wgt[u][r] = weight_pair(f);
-                        // entries are 16-byte aligned: one ds_read_b128, 8 lanes per LDS cycle
-                        v[u][r] = *static_cast<const Entry*>(
-                            __builtin_assume_aligned(row + k * (int)sizeof(Entry), 16));
+                        // entries are 16-byte aligned: one ds_read_b128 per group, 8 lanes per LDS cycle
+                        const char* at = row + k * (int)sizeof(HRow);
+#pragma unroll
+                        for (int gi = 0; gi < G; gi++)
+                            v[u][r].h[gi] = *static_cast<const HRow*>(
+                                __builtin_assume_aligned(at + gi * PLANE * (int)sizeof(HRow), 16));
                     }
                 }
 #pragma unroll
@@ -541,7 +546,12 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                     float f;
                     bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
                     const h2_t wgt = weight_pair(f);
-                    const Entry v = *static_cast<const Entry*>(__builtin_assume_aligned(row + k * (int)sizeof(Entry), 16));
+                    const char* at = row + k * (int)sizeof(HRow);
+                    Entry v;
+#pragma unroll
+                    for (int gi = 0; gi < G; gi++)
+                        v.h[gi] = *static_cast<const HRow*>(
+                            __builtin_assume_aligned(at + gi * PLANE * (int)sizeof(HRow), 16));
 #pragma unroll
                     for (int b = 0; b < B; b++)
                         acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.h[b / 4].v[b % 4], acc[r][b], false);
@@ -730,12 +740,16 @@ struct BpF16Variant {
 #define RISA_BPH(TX, TY, PPT, E, U, G, AMAX) \
     {TX, TY, PPT, E, U, G, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U, G>}
 // 16-pixel-wide tiles: the 16-byte LDS entries of a half-wave row then
-// stay clear of bank conflicts; G = 2 shares each pixel's weight over 8 slices
+// stay clear of bank conflicts; G = 2 shares each pixel's weight over 8
+// slices.  The LDS planes take G * 2 * E * threads * 16 B, so G = 2 pairs
+// with E * threads = 512 (32 KB), leaving room for several workgroups per CU.
 static const BpF16Variant kF16Variants[] = {
-    RISA_BPH(16, 16, 1, 4, 4, 1, 64), RISA_BPH(16, 16, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 2, 4, 2, 1, 64),
-    RISA_BPH(16, 8, 1, 4, 4, 1, 64),  RISA_BPH(16, 32, 4, 4, 2, 1, 64), RISA_BPH(16, 16, 1, 4, 4, 2, 64),
-    RISA_BPH(16, 32, 2, 4, 2, 2, 64), RISA_BPH(16, 8, 1, 4, 4, 2, 64),  RISA_BPH(16, 16, 2, 4, 2, 2, 64),
-    RISA_BPH(16, 16, 1, 4, 2, 2, 64),
+    RISA_BPH(16, 16, 1, 4, 4, 1, 64), RISA_BPH(16, 32, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 2, 2, 2, 1, 64),
+    RISA_BPH(16, 16, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 4, 4, 2, 1, 64), RISA_BPH(16, 8, 1, 4, 4, 1, 64),
+    RISA_BPH(16, 16, 1, 2, 4, 2, 64), RISA_BPH(16, 32, 2, 2, 2, 2, 64), RISA_BPH(16, 16, 2, 4, 2, 2, 64),
+    RISA_BPH(16, 32, 4, 4, 2, 2, 64), RISA_BPH(16, 8, 1, 4, 4, 2, 64),  RISA_BPH(16, 32, 4, 2, 2, 2, 64),
+    RISA_BPH(16, 16, 1, 2, 2, 2, 64), RISA_BPH(16, 16, 1, 1, 2, 4, 64), RISA_BPH(16, 32, 2, 1, 2, 4, 64),
+    RISA_BPH(16, 16, 2, 2, 2, 4, 64), RISA_BPH(16, 8, 1, 2, 2, 4, 64),
 };
 #undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
@@ -871,9 +885,10 @@ static void launch_f16x4(risa_bp_t h, int variant, const uint2* sino, float in_s
     BpParams p = make_params(h, v.tx, v.ty);
     p.out_scale = 1.0f / in_scale;
     const int threads = v.tx * v.ty / v.ppt;
-    const int entry = (int)sizeof(HRow) * v.g;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, entry);
-    const size_t lds = tile_lds(p.AE, p.W, entry);
+    // per group an LDS plane of two chunks of E * threads entries
+    if (p.W > v.e * threads) fail(RISA_ERR_UNSUPPORTED, "backprojection window exceeds the tile's staging");
+    p.AE = std::max(1, std::min(v.amax, v.e * threads / p.W));
+    const size_t lds = (size_t)v.g * 2 * v.e * threads * sizeof(HRow) + (size_t)3 * p.AE * 24;
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(ceil_div(slices, kF16Group), v.g));
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
@@ -945,39 +960,37 @@ extern "C" int risa_hip_bp_get_variant(risa_bp_t h, int* variant)
     });
 }
 
-// `slices` FP32 slices per timed launch stand for that many frames in flight
-// (the chain keeps several frames of one geometry in flight at once).
-extern "C" int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream)
+// Times every variant that fits the geometry with HIP events and keeps the
+// fastest: FP32 variants at s32 slices per launch (the frames a caller keeps
+// in flight), FP16 variants at s16 slices per launch; 0 skips a precision.
+static void autotune(risa_bp_t h, int s32, int s16, hipStream_t s)
 {
-    const hipStream_t s = as_stream(stream);
-    return guarded("risa_hip_bp_autotune_slices", [&] {
-        if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
-        DeviceGuard dg(h->device);
-        const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
-        const int B = std::max(8, slices);
-        DeviceBuffer<float> sino((size_t)P * D * slices);
-        DeviceBuffer<uint2> sino16((size_t)P * D * ceil_div(B, kF16Group));
-        DeviceBuffer<float> img((size_t)N * N * B);
-        RISA_HIP_CHECK(hipMemsetAsync(sino.ptr, 0, sino.count * sizeof(float), s));
-        RISA_HIP_CHECK(hipMemsetAsync(sino16.ptr, 0, sino16.count * sizeof(uint2), s));
-        hipEvent_t e0, e1;
-        RISA_HIP_CHECK(hipEventCreate(&e0));
-        RISA_HIP_CHECK(hipEventCreate(&e1));
-        auto time_it = [&](auto&& launch) {
-            launch();
-            RISA_HIP_CHECK(hipEventRecord(e0, s));
-            for (int it = 0; it < 5; it++) launch();
-            RISA_HIP_CHECK(hipEventRecord(e1, s));
-            RISA_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0;
-            RISA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            return ms;
-        };
+    DeviceGuard dg(h->device);
+    const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
+    DeviceBuffer<float> sino((size_t)P * D * std::max(1, s32));
+    DeviceBuffer<uint2> sino16((size_t)P * D * std::max(1, ceil_div(s16, kF16Group)));
+    DeviceBuffer<float> img((size_t)N * N * std::max(std::max(s32, s16), 1));
+    RISA_HIP_CHECK(hipMemsetAsync(sino.ptr, 0, sino.count * sizeof(float), s));
+    RISA_HIP_CHECK(hipMemsetAsync(sino16.ptr, 0, sino16.count * sizeof(uint2), s));
+    hipEvent_t e0, e1;
+    RISA_HIP_CHECK(hipEventCreate(&e0));
+    RISA_HIP_CHECK(hipEventCreate(&e1));
+    auto time_it = [&](auto&& launch) {
+        launch();
+        RISA_HIP_CHECK(hipEventRecord(e0, s));
+        for (int it = 0; it < 5; it++) launch();
+        RISA_HIP_CHECK(hipEventRecord(e1, s));
+        RISA_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        RISA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    try {
         float best = 1e30f;
-        for (int v = 0; v < kNumVariants; v++) {
+        for (int v = 0; s32 > 0 && v < kNumVariants; v++) {
             float ms;
             try {
-                ms = time_it([&] { launch_linear(h, v, sino.ptr, img.ptr, slices, s); });
+                ms = time_it([&] { launch_linear(h, v, sino.ptr, img.ptr, s32, s); });
             } catch (const Error& e) {
                 if (e.code != RISA_ERR_UNSUPPORTED) throw;
                 continue;   // this tile shape does not fit the geometry
@@ -985,23 +998,41 @@ extern "C" int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream
             if (ms < best) { best = ms; h->variant = v; }
         }
         best = 1e30f;
-        for (int v = 0; v < kNumF16Variants; v++) {
+        for (int v = 0; s16 > 0 && v < kNumF16Variants; v++) {
             float ms;
             try {
-                ms = time_it([&] { launch_f16x4(h, v, sino16.ptr, 1.0f, B, img.ptr, s); });
+                ms = time_it([&] { launch_f16x4(h, v, sino16.ptr, 1.0f, s16, img.ptr, s); });
             } catch (const Error& e) {
                 if (e.code != RISA_ERR_UNSUPPORTED) throw;
                 continue;
             }
             if (ms < best) { best = ms; h->variant_f16 = v; }
         }
+    } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+extern "C" int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream)
+{
+    return guarded("risa_hip_bp_autotune_slices", [&] {
+        if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
+        autotune(h, slices, std::max(8, slices), as_stream(stream));
     });
 }
 
-// Autotuner: times every tile shape on this geometry (FP32 single slice and
-// the FP16 batched form) with HIP events and keeps the fastest.
+extern "C" int risa_hip_bp_autotune_f16(risa_bp_t h, int slices, void* stream)
+{
+    return guarded("risa_hip_bp_autotune_f16", [&] {
+        if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
+        autotune(h, 0, slices, as_stream(stream));
+    });
+}
+
 extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
 {
     return risa_hip_bp_autotune_slices(h, 1, stream);

@@ -100,8 +100,9 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
                 h->par_buf.resize(par * max_frames);
                 h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
-            // tile shape tuned for the concurrency the chain runs the kernel at
-            ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
+            // tile shape tuned for the concurrency / batch the chain runs the kernel at
+            if (mode == RISA_CHAIN_PLAIN) ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
+            else ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
             delete h;

@@ -187,6 +187,9 @@ class Backprojection(_Handle):
     def autotune_slices(self, slices: int):
         check(load().risa_hip_bp_autotune_slices(self.h, slices, _stream()))
 
+    def autotune_f16(self, slices: int):
+        check(load().risa_hip_bp_autotune_f16(self.h, slices, _stream()))
+
     @property
     def variant(self) -> int:
         v = C.c_int()
