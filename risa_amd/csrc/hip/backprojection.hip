@@ -179,6 +179,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     __syncthreads();
 
     float s0r[E], s1r[E];
+    bool in0r[E], in1r[E];
     int kr[E];
     auto fetch = [&](int ch) {
         const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
@@ -188,6 +189,8 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             const int e = tid + q * NT;
             s0r[q] = 0.0f;
             s1r[q] = 0.0f;
+            in0r[q] = false;
+            in1r[q] = false;
             kr[q] = 0;
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
@@ -195,14 +198,26 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 const int2 ki = kin[a];
                 const int k = ki.y + w;
                 kr[q] = ki.x + w;
-                const long off = (long)(p0 + a) * p.D;
+                // unconditional loads at clamped indices (inside row p0 + a),
+                // zeroed outside [0, D); 32-bit offsets from the uniform base
+                const unsigned row = (unsigned)(p0 + a) * (unsigned)p.D;
+                const unsigned i0 = row + (unsigned)min(max(k, 0), p.D - 1);
+                const unsigned i1 = row + (unsigned)min(max(k + 1, 0), p.D - 1);
+                float v0, v1;
                 if (in_f16) {
-                    if (k >= 0 && k < p.D) s0r[q] = half_bits_to_float(S16[off + k]);
-                    if (k + 1 >= 0 && k + 1 < p.D) s1r[q] = half_bits_to_float(S16[off + k + 1]);
+                    const char* b = reinterpret_cast<const char*>(S16);
+                    v0 = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i0 * 2u));
+                    v1 = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i1 * 2u));
                 } else {
-                    if (k >= 0 && k < p.D) s0r[q] = S[off + k];
-                    if (k + 1 >= 0 && k + 1 < p.D) s1r[q] = S[off + k + 1];
+                    const char* b = reinterpret_cast<const char*>(S);
+                    v0 = *reinterpret_cast<const float*>(b + i0 * 4u);
+                    v1 = *reinterpret_cast<const float*>(b + i1 * 4u);
                 }
+                // zeroed at commit, so the loads stay in flight meanwhile
+                s0r[q] = v0;
+                s1r[q] = v1;
+                in0r[q] = (unsigned)k < (unsigned)p.D;
+                in1r[q] = (unsigned)(k + 1) < (unsigned)p.D;
             }
         }
     };
@@ -213,8 +228,9 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
             if (e < nae) {
-                const float d = s1r[q] - s0r[q];
-                buf[e] = make_float2(s0r[q] - (float)kr[q] * d, d);
+                const float s0 = in0r[q] ? s0r[q] : 0.0f, s1 = in1r[q] ? s1r[q] : 0.0f;
+                const float d = s1 - s0;
+                buf[e] = make_float2(s0 - (float)kr[q] * d, d);
             }
         }
     };
@@ -457,6 +473,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     __syncthreads();
 
     uint2 q0r[E][G], q1r[E][G];
+    bool in0r[E], in1r[E];
     auto fetch = [&](int ch) {
         const int p0 = ch * AE, nae = min(AE, p.P - p0) * W;
         const int2* kin = kinfo + (ch % 3) * AE;
@@ -468,15 +485,25 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
                 q0r[q][gi] = make_uint2(0u, 0u);
                 q1r[q][gi] = make_uint2(0u, 0u);
             }
+            in0r[q] = false;
+            in1r[q] = false;
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
                 const int k = kin[a].y + e - a * W;
+                // unconditional loads at clamped indices (inside row p0 + a),
+                // zeroed outside [0, D); 32-bit byte offsets from each group's base
+                const unsigned row = (unsigned)(p0 + a) * (unsigned)p.D;
+                const unsigned o0 = (row + (unsigned)min(max(k, 0), p.D - 1)) * 8u;
+                const unsigned o1 = (row + (unsigned)min(max(k + 1, 0), p.D - 1)) * 8u;
+                // zeroed at commit, so the loads stay in flight meanwhile
+                in0r[q] = (unsigned)k < (unsigned)p.D;
+                in1r[q] = (unsigned)(k + 1) < (unsigned)p.D;
 #pragma unroll
                 for (int gi = 0; gi < G; gi++) {
                     if (gi >= ng) break;
-                    const uint2* row = S + gi * gstride + (long)(p0 + a) * p.D;
-                    if (k >= 0 && k < p.D) q0r[q][gi] = row[k];
-                    if (k + 1 >= 0 && k + 1 < p.D) q1r[q][gi] = row[k + 1];
+                    const char* b = reinterpret_cast<const char*>(S + gi * gstride);
+                    q0r[q][gi] = *reinterpret_cast<const uint2*>(b + o0);
+                    q1r[q][gi] = *reinterpret_cast<const uint2*>(b + o1);
                 }
             }
         }
@@ -489,7 +516,10 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
             const int e = tid + q * NT;
             if (e < nae) {
 #pragma unroll
-                for (int gi = 0; gi < G; gi++) buf[gi * PLANE + e] = stage_pairs(q0r[q][gi], q1r[q][gi]);
+                for (int gi = 0; gi < G; gi++) {
+                    const uint2 z = make_uint2(0u, 0u);
+                    buf[gi * PLANE + e] = stage_pairs(in0r[q] ? q0r[q][gi] : z, in1r[q] ? q1r[q][gi] : z);
+                }
             }
         }
     };
@@ -795,8 +825,17 @@ struct risa_bp_s {
 
 using namespace risa;
 
+// The kernels address a slice's sinogram with 32-bit byte offsets (up to
+// 8 B per sample in the FP16 4-slice layout).
+static void check_sino_size(int P, int D)
+{
+    if ((long)P * D * 8 >= (1L << 31))
+        fail(RISA_ERR_UNSUPPORTED, "sinogram of %d x %d samples exceeds the kernels' 32-bit offsets", P, D);
+}
+
 static BpParams make_params(const risa_bp_s* h, int tx, int ty)
 {
+    check_sino_size(h->g.numberOfParallelProjections, h->g.numberOfParallelDetectors);
     BpParams p{};
     p.N = h->g.numberOfPixels;
     p.P = h->g.numberOfParallelProjections;
@@ -1136,6 +1175,7 @@ static void batched_layout(risa_bp_batched_s* h)
     const int threads = v.tx * v.ty / v.ppt;
     for (size_t i = 0; i < h->jobs.size(); i++) {
         const risa_bp_job& j = h->jobs[i];
+        check_sino_size(j.P, j.D);
         BpParams p{};
         p.N = j.N; p.P = j.P; p.D = j.D;
         p.cI = (int)(j.D * 0.5);
