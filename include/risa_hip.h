@@ -303,6 +303,11 @@ int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* plane
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
  * 2 filtered (FP32 modes only). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
+/* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces
+ * the autotuner's pick).  Only before the chain's first graph run, else
+ * RISA_ERR_UNSUPPORTED.  No reference counterpart. */
+int risa_hip_chain_set_variant(risa_chain_t h, int variant);
+int risa_hip_chain_get_variant(risa_chain_t h, int* variant);
 int risa_hip_chain_destroy(risa_chain_t h);
 
 #ifdef __cplusplus

@@ -189,6 +189,8 @@ _SIGS = {
     "risa_hip_chain_create": [C.POINTER(Geom), P, P, C.c_int, C.c_int, C.c_int, C.POINTER(P)],
     "risa_hip_chain_run": [P, P, P, C.c_int, P, C.c_int, P],
     "risa_hip_chain_intermediate": [P, C.c_int, C.POINTER(P)],
+    "risa_hip_chain_set_variant": [P, C.c_int],
+    "risa_hip_chain_get_variant": [P, C.POINTER(C.c_int)],
     "risa_hip_chain_destroy": [P],
     # synthetic source (include/risa_synth.h)
     "risa_synth_phantom": [C.c_uint64, C.c_double, C.POINTER(SynthOpts), P, C.c_int],

@@ -20,6 +20,7 @@
 #include "device_common.h"
 
 #include <algorithm>
+#include <vector>
 #include <cmath>
 
 #include "../host/tables.h"
@@ -70,6 +71,28 @@ __device__ __forceinline__ void bp_split(float t, int& k, float& f)
 {
     k = floor_to_int(t);
     f = __builtin_amdgcn_fractf(t);
+}
+
+// min(max(k, 0), hi) in one instruction (the compiler emits two).
+__device__ __forceinline__ int clamp_index(int k, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(k), "s"(hi));
+    return r;
+}
+
+// LDS byte addresses: the per-angle row address (table base included) is
+// staged once per chunk, so an update's LDS address is one v_lshl_add_u32.
+typedef float lds_f2 __attribute__((ext_vector_type(2)));
+typedef unsigned lds_u4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned lds_address(const void* p)
+{
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) char*)p;
+}
+template <class T>
+__device__ __forceinline__ T lds_load(unsigned addr)
+{
+    return *(const __attribute__((address_space(3))) T*)(size_t)addr;
 }
 
 // Per-tile preparation of one angle.
@@ -128,6 +151,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     const uint16_t* S16 = reinterpret_cast<const uint16_t*>(sino_v);
     const int W = p.W, AE = p.AE;
     float2* rows = reinterpret_cast<float2*>(smem);
+    const unsigned rbase = lds_address(rows);
     float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(float2));
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
@@ -170,7 +194,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
             const int row = (ch & 1) * AE * W + a * W;
-            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * 8));
+            inf[a] = make_float4(w.c, w.s, w.negT, __uint_as_float(rbase + (unsigned)((row - w.kmin) * 8)));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
     };
@@ -187,11 +211,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
-            s0r[q] = 0.0f;
-            s1r[q] = 0.0f;
-            in0r[q] = false;
-            in1r[q] = false;
-            kr[q] = 0;
+            // (entries e >= nae are neither loaded nor committed)
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
                 const int w = e - a * W;
@@ -200,9 +220,9 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 kr[q] = ki.x + w;
                 // unconditional loads at clamped indices (inside row p0 + a),
                 // zeroed outside [0, D); 32-bit offsets from the uniform base
-                const unsigned row = (unsigned)(p0 + a) * (unsigned)p.D;
-                const unsigned i0 = row + (unsigned)min(max(k, 0), p.D - 1);
-                const unsigned i1 = row + (unsigned)min(max(k + 1, 0), p.D - 1);
+                const unsigned row = __umul24((unsigned)(p0 + a), (unsigned)p.D);   // P, D < 2^24
+                const unsigned i0 = row + (unsigned)clamp_index(k, p.D - 1);
+                const unsigned i1 = row + (unsigned)clamp_index(k + 1, p.D - 1);
                 float v0, v1;
                 if (in_f16) {
                     const char* b = reinterpret_cast<const char*>(S16);
@@ -238,7 +258,6 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     fetch(0);
     commit(0);
     __syncthreads();
-    const char* rb = reinterpret_cast<const char*>(rows);
     for (int ch = 0; ch < nch; ch++) {
         const int p0 = ch * AE, na = min(AE, p.P - p0);
         const float4* inf = info + (ch % 3) * AE;
@@ -256,11 +275,12 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
-                    const int b = __float_as_int(in[u].w);
+                    const unsigned b = __float_as_uint(in[u].w);
 #pragma unroll
                     for (int r = 0; r < PPT; r++) {
                         t[u][r] = __builtin_fmaf(yp[r], in[u].y, xc);
-                        v[u][r] = *reinterpret_cast<const float2*>(rb + b + floor_to_int(t[u][r]) * 8);
+                        const lds_f2 q = lds_load<lds_f2>(b + (unsigned)floor_to_int(t[u][r]) * 8u);
+                        v[u][r] = make_float2(q.x, q.y);
                     }
                 }
 #pragma unroll
@@ -274,11 +294,12 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             for (; a < na; a++) {
                 const float4 in = inf[a];
                 const float xc = __builtin_fmaf(xp, in.x, in.z);
-                const int b = __float_as_int(in.w);
+                const unsigned b = __float_as_uint(in.w);
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
                     const float t = __builtin_fmaf(yp[r], in.y, xc);
-                    const float2 v = *reinterpret_cast<const float2*>(rb + b + floor_to_int(t) * 8);
+                    const lds_f2 q = lds_load<lds_f2>(b + (unsigned)floor_to_int(t) * 8u);
+                    const float2 v = make_float2(q.x, q.y);
                     acc[r] += v.x;
                     acc[r] = __builtin_fmaf(t, v.y, acc[r]);
                 }
@@ -410,6 +431,7 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int W = p.W, AE = p.AE;
     HRow* rows = reinterpret_cast<HRow*>(smem);
+    const unsigned rbase = lds_address(rows);
     float4* info = reinterpret_cast<float4*>(smem + (size_t)G * PLANE * sizeof(HRow));
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
@@ -464,7 +486,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
             const int row = (ch & 1) * AE * W + a * W;
-            inf[a] = make_float4(w.c, w.s, w.negT, __int_as_float((row - w.kmin) * (int)sizeof(HRow)));
+            inf[a] = make_float4(w.c, w.s, w.negT,
+                                 __uint_as_float(rbase + (unsigned)((row - w.kmin) * (int)sizeof(HRow))));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
     };
@@ -480,21 +503,21 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
+            // (entries e >= nae are neither loaded nor committed; groups
+            // gi >= ng are loaded as zeros)
 #pragma unroll
             for (int gi = 0; gi < G; gi++) {
                 q0r[q][gi] = make_uint2(0u, 0u);
                 q1r[q][gi] = make_uint2(0u, 0u);
             }
-            in0r[q] = false;
-            in1r[q] = false;
             if (e < nae) {
                 const int a = (int)(((float)e + 0.5f) * p.invW);
                 const int k = kin[a].y + e - a * W;
                 // unconditional loads at clamped indices (inside row p0 + a),
                 // zeroed outside [0, D); 32-bit byte offsets from each group's base
-                const unsigned row = (unsigned)(p0 + a) * (unsigned)p.D;
-                const unsigned o0 = (row + (unsigned)min(max(k, 0), p.D - 1)) * 8u;
-                const unsigned o1 = (row + (unsigned)min(max(k + 1, 0), p.D - 1)) * 8u;
+                const unsigned row = __umul24((unsigned)(p0 + a), (unsigned)p.D);   // P, D < 2^24
+                const unsigned o0 = (row + (unsigned)clamp_index(k, p.D - 1)) * 8u;
+                const unsigned o1 = (row + (unsigned)clamp_index(k + 1, p.D - 1)) * 8u;
                 // zeroed at commit, so the loads stay in flight meanwhile
                 in0r[q] = (unsigned)k < (unsigned)p.D;
                 in1r[q] = (unsigned)(k + 1) < (unsigned)p.D;
@@ -526,7 +549,6 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
     fetch(0);
     commit(0);
     __syncthreads();
-    const char* rb = reinterpret_cast<const char*>(rows);
     for (int ch = 0; ch < nch; ch++) {
         const int p0 = ch * AE, na = min(AE, p.P - p0);
         const float4* inf = info + (ch % 3) * AE;
@@ -542,19 +564,19 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
-                    const char* row = rb + __float_as_int(in[u].w);
+                    const unsigned row = __float_as_uint(in[u].w);
 #pragma unroll
                     for (int r = 0; r < PPT; r++) {
                         int k;
                         float f;
                         bp_split(__builtin_fmaf(yp[r], in[u].y, xc), k, f);
                         wgt[u][r] = weight_pair(f);
-                        // entries are 16-byte aligned: one ds_read_b128 per group, 8 lanes per LDS cycle
-                        const char* at = row + k * (int)sizeof(HRow);
+                        // 16-byte entries: one ds_read_b128 per group, 8 lanes per LDS cycle
+                        const unsigned at = row + (unsigned)k * (unsigned)sizeof(HRow);
 #pragma unroll
                         for (int gi = 0; gi < G; gi++)
-                            v[u][r].h[gi] = *static_cast<const HRow*>(
-                                __builtin_assume_aligned(at + gi * PLANE * (int)sizeof(HRow), 16));
+                            v[u][r].h[gi] = __builtin_bit_cast(
+                                HRow, lds_load<lds_u4>(at + gi * PLANE * (unsigned)sizeof(HRow)));
                     }
                 }
 #pragma unroll
@@ -569,19 +591,19 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
             for (; a < na; a++) {
                 const float4 in = inf[a];
                 const float xc = __builtin_fmaf(xp, in.x, in.z);
-                const char* row = rb + __float_as_int(in.w);
+                const unsigned row = __float_as_uint(in.w);
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
                     int k;
                     float f;
                     bp_split(__builtin_fmaf(yp[r], in.y, xc), k, f);
                     const h2_t wgt = weight_pair(f);
-                    const char* at = row + k * (int)sizeof(HRow);
+                    const unsigned at = row + (unsigned)k * (unsigned)sizeof(HRow);
                     Entry v;
 #pragma unroll
                     for (int gi = 0; gi < G; gi++)
-                        v.h[gi] = *static_cast<const HRow*>(
-                            __builtin_assume_aligned(at + gi * PLANE * (int)sizeof(HRow), 16));
+                        v.h[gi] = __builtin_bit_cast(HRow,
+                                                     lds_load<lds_u4>(at + gi * PLANE * (unsigned)sizeof(HRow)));
 #pragma unroll
                     for (int b = 0; b < B; b++)
                         acc[r][b] = __builtin_amdgcn_fdot2(wgt, v.h[b / 4].v[b % 4], acc[r][b], false);

@@ -247,7 +247,10 @@ Backprojection::Backprojection(const std::string& configFile)
         h_.push_back(h);
         glados::hip::ScopedDevice sd(w / lanes);
         // tile shape for the number of frames the device's lanes keep in flight
-        check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
+        int variant = 0;
+        if (w % lanes == 0) check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
+        check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "Backprojection variant");
+        check(risa_hip_bp_set_variant(h, variant), "Backprojection variant");
     }
     start("Backprojection", devices, lanes, pool, img_elems(g_),
           [this](int w, int, input_type& in, output_type& out, hipStream_t s) {

@@ -264,3 +264,13 @@ class Chain(_Handle):
         p = C.c_void_p()
         check(load().risa_hip_chain_intermediate(self.h, which, C.byref(p)))
         return p.value
+
+    @property
+    def variant(self) -> int:
+        v = C.c_int()
+        check(load().risa_hip_chain_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    @variant.setter
+    def variant(self, v: int):
+        check(load().risa_hip_chain_set_variant(self.h, v))
