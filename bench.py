@@ -196,10 +196,12 @@ def main():
         return ms
 
     plain = device.Chain(g, avg_ref, avg_dark, mode=risa_amd.CHAIN_PLAIN, max_frames=chunk)
+    log(f"plain chain: backprojector variant {plain.variant & 0xFF} (autotuned)")
     ms_plain = timed(plain, "plain FP32")
     img_plain_sample = img[:planes].cpu().numpy()
 
     mixed = device.Chain(g, avg_ref, avg_dark, mode=risa_amd.CHAIN_MIXED, max_frames=chunk)
+    log(f"mixed chain: FP16 backprojector variant {mixed.variant >> 8} (autotuned)")
     ms_mixed = timed(mixed, "mixed FP16/FP32")
     img_mixed_sample = img[:planes].cpu().numpy()
 

@@ -60,64 +60,6 @@ static void ok(int rc)
     if (rc != RISA_OK) fail(rc, "%s", risa_last_error());
 }
 
-static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img, hipStream_t s);
-
-// Picks the backprojector variant (FP32 for the plain chain, FP16 for the
-// mixed one) by timing the whole chain, eagerly, over min(max_frames, 16 or
-// 32) frames of zero counts with each variant that fits the geometry.
-static void tune_chain(risa_chain_t h)
-{
-    const risa_geom& g = h->g;
-    const bool plain = h->mode == RISA_CHAIN_PLAIN;
-    const int frames = std::min(h->max_frames, plain ? 16 : 32);
-    const size_t fan = (size_t)g.numberOfFanDetectors * g.numberOfFanProjections;
-    const size_t out = (size_t)g.numberOfPixels * g.numberOfPixels;
-    DeviceBuffer<uint16_t> raw(fan * frames);
-    DeviceBuffer<float> img(out * frames);
-    std::vector<int32_t> planes(frames);
-    for (int f = 0; f < frames; f++) planes[f] = f % g.numberOfPlanes;
-    RISA_HIP_CHECK(hipMemset(raw.ptr, 0, raw.count * sizeof(uint16_t)));
-    hipStream_t s;
-    RISA_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int base = 0, best = -1;
-    float best_ms = 1e30f;
-    try {
-        RISA_HIP_CHECK(hipEventCreate(&e0));
-        RISA_HIP_CHECK(hipEventCreate(&e1));
-        ok(risa_hip_bp_get_variant(h->bp, &base));
-        for (int v = 0;; v++) {
-            const int variant = plain ? ((base & ~0xff) | v) : ((base & 0xff) | (v << 8));
-            if (risa_hip_bp_set_variant(h->bp, variant) != RISA_OK) break;   // past the last variant
-            float ms = 0;
-            try {
-                enqueue(h, raw.ptr, planes.data(), frames, img.ptr, s);
-                RISA_HIP_CHECK(hipEventRecord(e0, s));
-                for (int it = 0; it < 2; it++) enqueue(h, raw.ptr, planes.data(), frames, img.ptr, s);
-                RISA_HIP_CHECK(hipEventRecord(e1, s));
-                RISA_HIP_CHECK(hipEventSynchronize(e1));
-                RISA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            } catch (const Error& e) {
-                if (e.code != RISA_ERR_UNSUPPORTED) throw;
-                RISA_HIP_CHECK(hipDeviceSynchronize());
-                continue;   // this tile shape does not fit the geometry
-            }
-            if (ms < best_ms) { best_ms = ms; best = variant; }
-        }
-        if (best < 0) fail(RISA_ERR_UNSUPPORTED, "no backprojector variant fits this geometry");
-        ok(risa_hip_bp_set_variant(h->bp, best));
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        (void)hipStreamDestroy(s);
-        throw;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-}
-
 extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float* avg_dark, int device,
                                      int mode, int max_frames, risa_chain_t* out)
 {
@@ -158,9 +100,12 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
                 h->par_buf.resize(par * max_frames);
                 h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
-            // backprojector tile shape: timed in the chain itself, where it
-            // shares the GPU with the other stages' kernels
-            tune_chain(h);
+            // backprojector tile shape, timed at the frames the chain keeps
+            // in flight (plain: one slice per lane) or at its batch (mixed);
+            // timing the chain itself per variant, eagerly, proved too noisy
+            // to rank the closest shapes
+            if (mode == RISA_CHAIN_PLAIN) ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
+            else ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
             delete h;
