@@ -792,16 +792,14 @@ struct BpF16Variant {
 #define RISA_BPH(TX, TY, PPT, E, U, G, AMAX) \
     {TX, TY, PPT, E, U, G, AMAX, bp_linear_f16_kernel<TX, TY, PPT, E, U, G>}
 // 16-pixel-wide tiles: the 16-byte LDS entries of a half-wave row then
-// stay clear of bank conflicts; G = 2 shares each pixel's weight over 8
-// slices.  The LDS planes take G * 2 * E * threads * 16 B, so G = 2 pairs
-// with E * threads = 512 (32 KB), leaving room for several workgroups per CU.
+// stay clear of bank conflicts; G groups of 4 slices share each pixel's
+// weight.  The LDS planes take G * 2 * E * threads * 16 B, at most 32 KB
+// here, leaving room for several workgroups per CU.
 static const BpF16Variant kF16Variants[] = {
-    RISA_BPH(16, 16, 1, 4, 4, 1, 64), RISA_BPH(16, 32, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 2, 2, 2, 1, 64),
-    RISA_BPH(16, 16, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 4, 4, 2, 1, 64), RISA_BPH(16, 8, 1, 4, 4, 1, 64),
-    RISA_BPH(16, 16, 1, 2, 4, 2, 64), RISA_BPH(16, 32, 2, 2, 2, 2, 64), RISA_BPH(16, 16, 2, 4, 2, 2, 64),
-    RISA_BPH(16, 32, 4, 4, 2, 2, 64), RISA_BPH(16, 8, 1, 4, 4, 2, 64),  RISA_BPH(16, 32, 4, 2, 2, 2, 64),
-    RISA_BPH(16, 16, 1, 2, 2, 2, 64), RISA_BPH(16, 16, 1, 1, 2, 4, 64), RISA_BPH(16, 32, 2, 1, 2, 4, 64),
-    RISA_BPH(16, 16, 2, 2, 2, 4, 64), RISA_BPH(16, 8, 1, 2, 2, 4, 64),
+    RISA_BPH(16, 32, 2, 4, 2, 1, 64), RISA_BPH(16, 32, 2, 2, 2, 1, 64), RISA_BPH(16, 32, 4, 4, 2, 1, 64),
+    RISA_BPH(16, 16, 1, 2, 4, 2, 64), RISA_BPH(16, 32, 2, 2, 2, 2, 64), RISA_BPH(16, 32, 4, 4, 2, 2, 64),
+    RISA_BPH(16, 32, 4, 2, 2, 2, 64), RISA_BPH(16, 16, 1, 2, 2, 2, 64), RISA_BPH(16, 16, 1, 1, 2, 4, 64),
+    RISA_BPH(16, 8, 1, 1, 2, 8, 64),
 };
 #undef RISA_BPH
 static constexpr int kNumF16Variants = sizeof(kF16Variants) / sizeof(kF16Variants[0]);
