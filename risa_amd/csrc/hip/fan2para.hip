@@ -113,21 +113,47 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
     const float fgm[4] = {__uint_as_float(fgb.x), __uint_as_float(fgb.y), __uint_as_float(fgb.z),
                           __uint_as_float(fgb.w)};
     const int n = fg.count[grp], first = fg.start[grp];
+    // This thread's share of the patches, the same for every frame: element
+    // t = tid + j * 256 (j < 2, as a patch holds at most 512 samples) of slot
+    // sl sits at sample offset goff[sl][j] of a frame and at LDS slot
+    // sl * kPatchMax + t.  The next frame's samples are loaded into registers
+    // while the current frame is interpolated.
+    constexpr int kPer = kPatchMax / 256;
+    int goff[4][kPer];
+    bool have[4][kPer];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+        const int4 b = box[sl];
+        const int cnt = b.z > 0 ? b.z * b.w : 0;
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int t = threadIdx.x + j * 256;
+            have[sl][j] = t < cnt;
+            const int r = have[sl][j] ? t / b.w : 0;
+            goff[sl][j] = have[sl][j] ? (b.x + r) * Df + b.y + (t - r * b.w) : 0;
+        }
+    }
+    float pf[4][kPer];
+    auto load_frame = [&](int u) {
+        const float* fr = fan + (long)fg.frame[first + u] * Df * Pf;
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+            for (int j = 0; j < kPer; j++)
+                if (have[sl][j]) pf[sl][j] = fr[goff[sl][j]];
+    };
+    if (n > 0) load_frame(0);
     for (int u = 0; u < n; u++) {
         const int f = fg.frame[first + u];
         const float* fr = fan + (long)f * Df * Pf;
         if (u > 0) __syncthreads();   // the previous frame's patches are consumed
 #pragma unroll
-        for (int sl = 0; sl < 4; sl++) {
-            const int4 b = box[sl];
-            if (b.z <= 0) continue;
-            const int cnt = b.z * b.w;
-            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-                const int r = t / b.w, c = t - r * b.w;
-                patch[sl * kPatchMax + t] = fr[(long)(b.x + r) * Df + b.y + c];
-            }
-        }
+        for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+            for (int j = 0; j < kPer; j++)
+                if (have[sl][j]) patch[sl * kPatchMax + threadIdx.x + j * 256] = pf[sl][j];
         __syncthreads();
+        if (u + 1 < n) load_frame(u + 1);
         if (!inside) continue;
         float z[4];
 #pragma unroll

@@ -89,6 +89,26 @@ def test_fan2para_matches_oracle(risa, orc, geom_c0, frames_c0, cuda):
     assert rel_rmse(got, ref64) < 5e-7
 
 
+def test_fan2para_batch_equals_single(risa, geom_c1, cuda):
+    """A batched launch (frames grouped by plane, each workgroup looping over
+    its group's frames) gives every frame exactly its single-frame result."""
+    import torch
+    from risa_amd import device, synth
+
+    print("case: fan2para batch C1, 9 frames over 2 planes", flush=True)
+    g = geom_c1
+    planes = np.array([0, 1, 1, 0, 1, 0, 0, 1, 0], np.int32)
+    rng = np.random.default_rng(17)
+    fan = torch.from_numpy(rng.standard_normal((len(planes),) + g.fan_shape).astype(np.float32)).to(cuda)
+    f2p = device.Fan2Para(g)
+    outb = torch.empty((len(planes),) + g.par_shape, dtype=torch.float32, device=cuda)
+    f2p.batch(fan, planes, outb)
+    for f in range(len(planes)):
+        one = torch.empty(g.par_shape, dtype=torch.float32, device=cuda)
+        f2p(fan[f], int(planes[f]), one)
+        assert torch.equal(one, outb[f]), f"frame {f}"
+
+
 def test_filter_fft_matches_oracle(risa, orc, geom_c0, cuda):
     import torch
     from risa_amd import device
