@@ -79,8 +79,9 @@ def kernel_only(args, g, dev):
     rng = np.random.default_rng(1)
     with torch.cuda.stream(stream):
         if args.kernel in ("bp", "bp16"):
+            # every dispatch of the run has the timed shape (the autotuner's
+            # included), so per-kernel counter averages are per such launch
             bp = device.Backprojection(g)
-            bp.autotune()
             if args.kernel == "bp":
                 S = BP_SLICES_IN_FLIGHT
                 bp.autotune_slices(S)
@@ -329,12 +330,14 @@ def main():
 
     # HBM bytes per launch of the backprojector from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size)
-    traffic, traffic_kernel = None, None
+    # (per kernel: the profiled run dispatches every variant while autotuning)
+    traffic, traffic_kernel = None, device.bp_kernel_name(False, bp_variant)
     tf = ROOT / "profiles" / "bp_c1_hbm_traffic.json"
     if tf.exists():
         tj = json.loads(tf.read_text())
-        if "bp" in tj:
-            traffic, traffic_kernel = tj["bp"]["bytes"], tj["bp"]["kernel"]
+        per = tj.get("bp_kernels", {})
+        if traffic_kernel in per:
+            traffic = per[traffic_kernel]
 
     plain_value = total / (ms_plain / 1e3)
     mixed_value = total / (ms_mixed / 1e3)

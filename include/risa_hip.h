@@ -244,6 +244,11 @@ int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* st
  * FP16 one; both start at 0 and are chosen by the autotune calls below. */
 int risa_hip_bp_set_variant(risa_bp_t h, int variant);
 int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
+/* Tile shape of a variant (f16 = 0: FP32 kernel, 1: FP16 kernel):
+ * shape = {tile x, tile y, pixels per thread, staged entries per thread,
+ * angles per step, groups of 4 slices (1 for FP32), max angles per chunk};
+ * RISA_ERR_ARG past the last variant.  (Tooling; no reference counterpart.) */
+int risa_hip_bp_variant_shape(int f16, int variant, int shape[7]);
 /* Times every variant that fits the geometry and keeps the fastest: FP32
  * variants at 1 slice per launch, FP16 variants at 8. */
 int risa_hip_bp_autotune(risa_bp_t h, void* stream);

@@ -176,6 +176,7 @@ _SIGS = {
     "risa_hip_bp_autotune": [P, P],
     "risa_hip_bp_autotune_slices": [P, C.c_int, P],
     "risa_hip_bp_autotune_f16": [P, C.c_int, P],
+    "risa_hip_bp_variant_shape": [C.c_int, C.c_int, C.POINTER(C.c_int)],
     "risa_hip_bp_destroy": [P],
     "risa_hip_mask_create": [C.POINTER(Geom), C.c_int, C.POINTER(P)],
     "risa_hip_mask_run": [P, P, P],

@@ -1011,6 +1011,24 @@ extern "C" int risa_hip_bp_set_variant(risa_bp_t h, int variant)
     });
 }
 
+extern "C" int risa_hip_bp_variant_shape(int f16, int variant, int shape[7])
+{
+    return guarded("risa_hip_bp_variant_shape", [&] {
+        if (!shape || variant < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (f16) {
+            if (variant >= kNumF16Variants) fail(RISA_ERR_ARG, "variant out of range");
+            const BpF16Variant& v = kF16Variants[variant];
+            const int s[7] = {v.tx, v.ty, v.ppt, v.e, v.u, v.g, v.amax};
+            std::copy(s, s + 7, shape);
+        } else {
+            if (variant >= kNumVariants) fail(RISA_ERR_ARG, "variant out of range");
+            const BpVariant& v = kVariants[variant];
+            const int s[7] = {v.tx, v.ty, v.ppt, v.e, v.u, 1, v.amax};
+            std::copy(s, s + 7, shape);
+        }
+    });
+}
+
 extern "C" int risa_hip_bp_get_variant(risa_bp_t h, int* variant)
 {
     return guarded("risa_hip_bp_get_variant", [&] {

@@ -148,6 +148,16 @@ class Filter(_Handle):
         return t
 
 
+def bp_kernel_name(f16: bool, variant: int) -> str:
+    """The backprojection kernel a variant launches, as profilers name it."""
+    shape = (C.c_int * 7)()
+    check(load().risa_hip_bp_variant_shape(1 if f16 else 0, variant, shape))
+    tx, ty, ppt, e, u, g, _ = list(shape)
+    if f16:
+        return f"risa::bp_linear_f16_kernel<{tx}, {ty}, {ppt}, {e}, {u}, {g}>"
+    return f"risa::bp_linear_kernel<{tx}, {ty}, {ppt}, {e}, {u}>"
+
+
 class Backprojection(_Handle):
     _destroy = "risa_hip_bp_destroy"
 
