@@ -267,6 +267,37 @@ def test_chain_plain_matches_fp64_oracle(risa, orc, geom_c0, frames_c0, cuda):
         _sync()
 
 
+def test_chain_variant_override(risa, orc, geom_c0, frames_c0, cuda):
+    """The chain's backprojector variant can replace the autotuner's pick
+    before the first graph run (each tile shape stays within the plain
+    chain's tolerance) and is refused once graphs hold the old launches."""
+    import torch
+    from risa_amd import _lib, device
+
+    print("case: chain variant override C0", flush=True)
+    g, fs = geom_c0, frames_c0
+    oc = orc.Chain(g, fs.avg_ref, fs.avg_dark)
+    m = disk_mask(g.numberOfPixels)
+    raw = _t(fs.raw[:2], cuda)
+    planes = fs.planes[:2]
+    refs = [oc.f64(fs.raw[f], int(planes[f])) for f in range(2)]
+    for v in (0, 2):
+        ch = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_PLAIN, max_frames=2)
+        ch.variant = (ch.variant & ~0xFF) | v
+        assert ch.variant & 0xFF == v
+        img = torch.empty((2,) + g.img_shape, dtype=torch.float32, device=cuda)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            ch(raw, planes, img, use_graph=True)
+        s.synchronize()
+        got = img.cpu().numpy()
+        for f in range(2):
+            assert rel_rmse(got[f], refs[f], m) < 1e-5, (v, f)
+        with pytest.raises(_lib.RisaError):
+            ch.variant = v
+        ch.close()
+
+
 def test_chain_mixed_within_2e4(risa, orc, geom_c0, frames_c0, cuda):
     import torch
     from risa_amd import device

@@ -131,6 +131,39 @@ def test_status_codes_cross_the_boundary(risa):
     assert lib.risa_last_error()
 
 
+def test_backprojector_variant_shapes(risa):
+    """The variant table the autotuner walks: every shape is well formed (a
+    16-pixel-wide FP16 tile, whole pixel columns per thread, FP16 LDS planes
+    within 32 KB), indices past the end are refused, and the kernel names
+    the profiler reports follow from the shapes."""
+    from risa_amd import _lib, device
+
+    lib = _lib.load()
+    for f16 in (0, 1):
+        shapes = []
+        while True:
+            s = (C.c_int * 7)()
+            rc = lib.risa_hip_bp_variant_shape(f16, len(shapes), s)
+            if rc:
+                assert rc == 1
+                break
+            shapes.append(tuple(s))
+        assert len(shapes) >= 4
+        assert len(set(shapes)) == len(shapes)
+        for tx, ty, ppt, e, u, g, amax in shapes:
+            assert ty % ppt == 0 and tx * ty // ppt <= 1024 and (tx * ty // ppt) % 64 == 0
+            assert e >= 1 and u >= 1 and 1 <= amax <= 64
+            if f16:
+                assert tx == 16 and g in (1, 2, 4, 8)
+                assert g * 2 * e * (tx * ty // ppt) * 16 <= 32 * 1024
+            else:
+                assert g == 1
+        name = device.bp_kernel_name(bool(f16), 0)
+        tx, ty, ppt, e, u, g, _ = shapes[0]
+        assert name.endswith(f"<{tx}, {ty}, {ppt}, {e}, {u}" + (f", {g}>" if f16 else ">"))
+    assert lib.risa_hip_bp_variant_shape(0, -1, (C.c_int * 7)()) == 1
+
+
 def test_synth_is_deterministic(risa, geom_c0):
     from risa_amd import synth
 
