@@ -418,41 +418,44 @@ __global__ void __launch_bounds__(512) dft_gemm_split_kernel(const float* __rest
     const bool vec = (D % 8) == 0;
 
     float16_t acc0 = {}, acc1 = {};
-    for (int k0 = 0; k0 < D; k0 += GB_K) {
+    // the next k-tile's global loads are in flight while this one's MFMAs run
+    float av[8];
+    half8_t bh, bl;
+    auto load = [&](int k0) {
         const int gk = k0 + sc8;
-        {
-            float v[8];
-            if (xrow >= 0 && vec && gk + 8 <= D) {
-                const float4* src = reinterpret_cast<const float4*>(X + xrow * D + gk);
-                const float4 a = src[0], b = src[1];
-                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-                v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-            } else {
+        if (xrow >= 0 && vec && gk + 8 <= D) {
+            const float4* src = reinterpret_cast<const float4*>(X + xrow * D + gk);
+            const float4 a = src[0], b = src[1];
+            av[0] = a.x; av[1] = a.y; av[2] = a.z; av[3] = a.w;
+            av[4] = b.x; av[5] = b.y; av[6] = b.z; av[7] = b.w;
+        } else {
 #pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = (xrow >= 0 && gk + u < D) ? X[xrow * D + gk + u] : 0.0f;
+            for (int u = 0; u < 8; u++) av[u] = (xrow >= 0 && gk + u < D) ? X[xrow * D + gk + u] : 0.0f;
+        }
+        if (cn < D && vec && gk + 8 <= D) {
+            bh = *reinterpret_cast<const half8_t*>(Ch + (long)cn * D + gk);
+            bl = *reinterpret_cast<const half8_t*>(Cl + (long)cn * D + gk);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const bool ok = cn < D && gk + u < D;
+                bh[u] = ok ? Ch[(long)cn * D + gk + u] : (_Float16)0.0f;
+                bl[u] = ok ? Cl[(long)cn * D + gk + u] : (_Float16)0.0f;
             }
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < D; k0 += GB_K) {
+        {
             half8_t hi, lo;
-            split8(v, sx, hi, lo);
+            split8(av, sx, hi, lo);
             *reinterpret_cast<half8_t*>(&Ah[sr][sc8]) = hi;
             *reinterpret_cast<half8_t*>(&Al[sr][sc8]) = lo;
-        }
-        {
-            half8_t hi, lo;
-            if (cn < D && vec && gk + 8 <= D) {
-                hi = *reinterpret_cast<const half8_t*>(Ch + (long)cn * D + gk);
-                lo = *reinterpret_cast<const half8_t*>(Cl + (long)cn * D + gk);
-            } else {
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const bool ok = cn < D && gk + u < D;
-                    hi[u] = ok ? Ch[(long)cn * D + gk + u] : (_Float16)0.0f;
-                    lo[u] = ok ? Cl[(long)cn * D + gk + u] : (_Float16)0.0f;
-                }
-            }
-            *reinterpret_cast<half8_t*>(&Bh[sr][sc8]) = hi;
-            *reinterpret_cast<half8_t*>(&Bl[sr][sc8]) = lo;
+            *reinterpret_cast<half8_t*>(&Bh[sr][sc8]) = bh;
+            *reinterpret_cast<half8_t*>(&Bl[sr][sc8]) = bl;
         }
         __syncthreads();
+        if (k0 + GB_K < D) load(k0 + GB_K);
 #pragma unroll
         for (int kk = 0; kk < GB_K; kk += 16) {
             const int r = lane % 32, kq = kk + (lane / 32) * 8;
@@ -667,7 +670,6 @@ extern "C" int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, voi
         if (h->mode == RISA_FILTER_MODE_FFT32) {
             launch_fft(h, sino, sino, nullptr, 1.0f, rows, as_stream(stream));
         } else {
-            // GEMM reads rows of other tiles: stage out of place, then copy back
             // the GEMM reads whole rows of X per output tile: filter into a
             // staging buffer (grown on first use), then copy back
             const size_t n = (size_t)rows * h->g.numberOfParallelDetectors;
