@@ -318,6 +318,37 @@ def test_chain_mixed_within_2e4(risa, orc, geom_c0, frames_c0, cuda):
         assert err < 2e-4
 
 
+def test_chain_mixed_large_call(risa, orc, geom_c0, frames_c0, cuda):
+    """A mixed call of 53 frames (13 whole 4-slice groups and a partial one):
+    eager and graph-replayed runs give identical images, each frame equal to
+    the same frame run alone."""
+    import torch
+    from risa_amd import device
+
+    print("case: chain mixed C0, 53 frames per call", flush=True)
+    g, fs = geom_c0, frames_c0
+    n = 53
+    idx = np.arange(n) % len(fs.planes)
+    raw = _t(np.ascontiguousarray(fs.raw[idx]), cuda)
+    planes = fs.planes[idx]
+    ch = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_MIXED, max_frames=n)
+    img = torch.empty((n,) + g.img_shape, dtype=torch.float32, device=cuda)
+    ch(raw, planes, img)
+    _sync()
+    s = torch.cuda.Stream()
+    for rep in range(2):
+        img2 = torch.full_like(img, float("nan"))
+        with torch.cuda.stream(s):
+            ch(raw, planes, img2, use_graph=True)
+        s.synchronize()
+        assert torch.equal(img, img2), rep
+    one = torch.empty((1,) + g.img_shape, dtype=torch.float32, device=cuda)
+    for f in (0, 17, 52):
+        ch(raw[f:f + 1], planes[f:f + 1], one)
+        _sync()
+        assert torch.equal(one[0], img[f]), f
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("D,P,slices", [(512, 1000, 3), (256, 500, 5), (500, 37, 2), (768, 130, 4)])
 def test_filter_gemm_split_matches_fp64(risa, orc, geom_c0, cuda, D, P, slices):
