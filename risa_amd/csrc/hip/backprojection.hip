@@ -38,7 +38,27 @@ struct BpParams {
     float invW;
     float out_scale;     // multiplies sum * norm (1/in_scale for FP16 input)
     int AE;              // angles per staged chunk
+    int zx, zy;          // log2 of the tile grid's width / height for the XCD-aware order, or -1
 };
+
+// The tile of workgroup b.  Workgroups are dealt round-robin to the chip's 8
+// XCDs, each with its own L2, so workgroup b runs on XCD b % 8.  On a
+// power-of-two tile grid the tiles of one XCD are a contiguous range of the
+// Z (Morton) order -- a compact block of the image -- which shrinks the union
+// of their sinogram windows, i.e. that XCD's L2 footprint and the HBM
+// re-reads; other grids keep the row-major order.
+__device__ __forceinline__ int2 bp_tile_xy(int b, const BpParams& p)
+{
+    if (p.zx < 0) return make_int2(b % p.tilesX, b / p.tilesX);
+    const int per = 1 << (p.zx + p.zy - 3);   // tiles per XCD (grid >= 8 tiles)
+    const int r = (b & 7) * per + (b >> 3);
+    int x = 0, y = 0, bit = 0;
+    for (int i = 0; i < p.zx || i < p.zy; i++) {
+        if (i < p.zx) x |= ((r >> bit++) & 1) << i;
+        if (i < p.zy) y |= ((r >> bit++) & 1) << i;
+    }
+    return make_int2(x, y);
+}
 
 // Detector coordinate of a pixel; the same operation sequence everywhere
 // (backprojector, its window bounds, the adjoint projector).
@@ -323,7 +343,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __
                                                                  const float* __restrict__ sinT, BpParams p)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
+    const int2 txy = bp_tile_xy(blockIdx.x, p);
+    const int x0 = txy.x * TX, y0 = txy.y * TY;
     const long slice = blockIdx.y;
     bp_tile<TX, TY, PPT, E, U>(sino + slice * (long)p.P * p.D, img + slice * (long)p.N * p.N, cosT, sinT, p, x0,
                                y0, 0, smem);
@@ -437,7 +458,8 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int x0 = (blockIdx.x % p.tilesX) * TX, y0 = (blockIdx.x / p.tilesX) * TY;
+    const int2 txy = bp_tile_xy(blockIdx.x, p);
+    const int x0 = txy.x * TX, y0 = txy.y * TY;
     const int g0 = blockIdx.y * G;
     const int ngroups = (slices + kF16Group - 1) / kF16Group;
     const int ng = min(G, ngroups - g0);
@@ -853,6 +875,19 @@ static void check_sino_size(int P, int D)
         fail(RISA_ERR_UNSUPPORTED, "sinogram of %d x %d samples exceeds the kernels' 32-bit offsets", P, D);
 }
 
+// XCD-aware tile order on power-of-two grids of at least 8 tiles.
+static void set_tile_order(BpParams& p, int tilesY)
+{
+    auto log2_exact = [](int v) {
+        int l = 0;
+        while ((1 << l) < v) l++;
+        return (1 << l) == v ? l : -1;
+    };
+    p.zx = log2_exact(p.tilesX);
+    p.zy = log2_exact(tilesY);
+    if (p.zx < 0 || p.zy < 0 || p.zx + p.zy < 3) p.zx = p.zy = -1;
+}
+
 static BpParams make_params(const risa_bp_s* h, int tx, int ty)
 {
     check_sino_size(h->g.numberOfParallelProjections, h->g.numberOfParallelDetectors);
@@ -868,6 +903,7 @@ static BpParams make_params(const risa_bp_s* h, int tx, int ty)
     p.maskingValue = (float)h->g.maskingValue;
     p.W = window_entries(tx, ty, p.scale);
     p.tilesX = ceil_div(p.N, tx);
+    set_tile_order(p, ceil_div(p.N, ty));
     p.invW = 1.0f / (float)p.W;
     p.out_scale = 1.0f;
     return p;
@@ -1224,6 +1260,7 @@ static void batched_layout(risa_bp_batched_s* h)
         p.maskingValue = j.maskingValue;
         p.W = window_entries(v.tx, v.ty, p.scale);
         p.tilesX = ceil_div(j.N, v.tx);
+        p.zx = p.zy = -1;   // the persistent kernel takes tiles from its cost-sorted list
         const int tilesY = ceil_div(j.N, v.ty);
         p.invW = 1.0f / (float)p.W;
         p.out_scale = j.dtype == RISA_F16 ? 1.0f / j.in_scale : 1.0f;
