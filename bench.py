@@ -127,6 +127,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the streamed C++ stage pipeline leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (96 geometries) leg")
     ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
     ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
@@ -279,6 +280,48 @@ def main():
         log(f"accuracy vs FP64 oracle: {rmse}")
         oc.close()
 
+    # Uneven batch (the 96-reconstruction workload: 256^2..768^2 grids,
+    # 360..1000 angles, 256..768 detectors, FP32 and FP16 sinograms) in one
+    # persistent launch, timed with HIP events; updates counted as N^2 * P.
+    uneven = None
+    if not args.no_uneven:
+        try:
+            Ns, Ps = (256, 384, 512, 768), (360, 500, 720, 1000)
+            rng_u = np.random.default_rng(96)
+            ujobs, upd = [], 0
+            for i in range(96):
+                Nj, Pj, Dj = Ns[i % 4], Ps[(i // 4) % 4], int(256 + (i * 37) % 513)
+                f16 = (i // 16) % 2
+                s_dev = torch.randn((Pj, Dj), device=dev) * 0.01
+                if f16:
+                    s_dev = (s_dev * 256.0).half()
+                ujobs.append(dict(sino=s_dev, img=torch.empty((Nj, Nj), device=dev), N=Nj, P=Pj, D=Dj, dtype=f16,
+                                  in_scale=256.0 if f16 else 1.0, rotationOffset=float(rng_u.uniform(-45, 45)),
+                                  mask=int(i % 3 != 0), maskingValue=0.0))
+                upd += Nj * Nj * Pj
+            with torch.cuda.stream(stream):
+                bb = device.BatchedBackprojection(ujobs)
+                bb.autotune()
+                for _ in range(2):
+                    bb()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(10):
+                    bb()
+                e1.record(stream)
+            stream.synchronize()
+            ums = e0.elapsed_time(e1) / 10
+            utf = FLOP_PER_UPDATE * upd / (ums * 1e-3) / 1e12
+            uneven = {"jobs": 96, "ms_per_launch": round(ums, 3), "jobs_per_s": round(96 / (ums * 1e-3), 1),
+                      "updates_per_launch": upd, "achieved_tflops": round(utf, 2),
+                      "frac_of_fp32_peak": round(utf / FP32_PEAK_TFLOPS, 4),
+                      "what": "bp_persistent_kernel, 96 uneven jobs (48 FP32 + 48 FP16 sinograms) in one launch"}
+            log(f"uneven batch: {uneven}")
+            bb.close()
+            del ujobs
+        except Exception as exc:  # reported, not fatal
+            log(f"uneven batch skipped: {exc}")
+
     # PCIe-inclusive rate: the C++ stage pipeline (H2D -> ... -> D2H) streaming
     # pinned host frames; reported beside the device-resident value, never as it.
     streamed = None
@@ -373,6 +416,7 @@ def main():
         "bp_hbm": {"achieved": round(bp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(bp_gbs / HBM_PEAK_GBS, 5), "bytes_per_launch": bp_bytes},
         "bp_f16": bp16,
+        "uneven_batch": uneven,
         "streamed_pipeline": streamed,
         "cpu_baseline": cpu,
     }
