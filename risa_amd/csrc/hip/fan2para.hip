@@ -31,8 +31,11 @@ struct FrameGroups {
     int32_t frame[kMaxFramesPerLaunch];   // frame offsets within the launch
 };
 
-__device__ __forceinline__ float rebin_ray(const float* __restrict__ fan, uint32_t idx, float ft, float fg, int Df,
-                                           int Pf)
+// (Out of line: a slot gathers from global memory in a few per cent of the
+// blocks only; inlined into the four unrolled slots it costs the kernel a
+// third more registers, i.e. occupancy.)
+__device__ __attribute__((noinline)) float rebin_ray(const float* __restrict__ fan, uint32_t idx, float ft, float fg,
+                                                     int Df, int Pf)
 {
     const int tb = (int)(idx & 0xFFFFu);
     const int gb = (int)((idx >> 16) & 0x7FFFu);
