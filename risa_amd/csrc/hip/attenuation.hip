@@ -22,6 +22,8 @@ struct PlaneList {
     int32_t p[64];
 };
 
+constexpr long kAttGrid = 256;   // workgroups per frame (grid-stride beyond)
+
 __global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __restrict__ raw,
                                                           const float* __restrict__ refm,
                                                           const float* __restrict__ dark,
@@ -31,32 +33,34 @@ __global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __rest
     const int f = blockIdx.y;
     const int plane = planes.p[f];
     const long n = (long)Df * rows;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // one thread: 4 detectors of one row
-    if (q >= (long)quads_per_row * rows) return;
-    const int y = (int)(q / quads_per_row);
-    const int x = (int)(q - (long)y * quads_per_row) * 4;
-    const long e = (long)y * Df + x;
     const uint16_t* r = raw + (long)f * n;
     const float* rm = refm + (long)plane * n;
     const float* dk = dark + (long)plane * Df;
     float* o = out + (long)f * n;
     const float temp = 1.0e-5f;   // (float)pow(10,-5)
-
-    const ushort4 v = *reinterpret_cast<const ushort4*>(r + e);
-    const float4 d = *reinterpret_cast<const float4*>(rm + e);
-    const float4 dkv = *reinterpret_cast<const float4*>(dk + x);
-    const float vin[4] = {(float)v.x, (float)v.y, (float)v.z, (float)v.w};
-    const float den[4] = {d.x, d.y, d.z, d.w};
-    const float drk[4] = {dkv.x, dkv.y, dkv.z, dkv.w};
-    float res[4];
+    // one iteration: 4 detectors of one row; at most kAttGrid workgroups
+    // per frame stride over it
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < (long)quads_per_row * rows;
+         q += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(q / quads_per_row);
+        const int x = (int)(q - (long)y * quads_per_row) * 4;
+        const long e = (long)y * Df + x;
+        const ushort4 v = *reinterpret_cast<const ushort4*>(r + e);
+        const float4 d = *reinterpret_cast<const float4*>(rm + e);
+        const float4 dkv = *reinterpret_cast<const float4*>(dk + x);
+        const float vin[4] = {(float)v.x, (float)v.y, (float)v.z, (float)v.w};
+        const float den[4] = {d.x, d.y, d.z, d.w};
+        const float drk[4] = {dkv.x, dkv.y, dkv.z, dkv.w};
+        float res[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float num = vin[k] - drk[k];
-        if (num < temp) num = temp;
-        const float m = den[k] > 0.0f ? 1.0f : 0.0f;
-        res[k] = -logf(num / fabsf(den[k])) * m;
+        for (int k = 0; k < 4; k++) {
+            float num = vin[k] - drk[k];
+            if (num < temp) num = temp;
+            const float m = den[k] > 0.0f ? 1.0f : 0.0f;
+            res[k] = -logf(num / fabsf(den[k])) * m;
+        }
+        *reinterpret_cast<float4*>(o + e) = make_float4(res[0], res[1], res[2], res[3]);
     }
-    *reinterpret_cast<float4*>(o + e) = make_float4(res[0], res[1], res[2], res[3]);
 }
 
 // Any D_f: one element per thread.
@@ -142,7 +146,7 @@ static void launch_attenuation(risa_attenuation_t h, const uint16_t* raw, const 
         const bool vec = (Df % 4 == 0) && ((uintptr_t)r % 8 == 0) && ((uintptr_t)o % 16 == 0);
         if (vec) {
             const int quads = Df / 4;
-            dim3 grid(ceil_div((long)quads * Pf, 256), nf);
+            dim3 grid((unsigned)std::min<long>(ceil_div((long)quads * Pf, 256), kAttGrid), nf);
             hipLaunchKernelGGL(attenuation_kernel, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df, quads,
                                Pf, pl);
         } else {
