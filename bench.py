@@ -127,7 +127,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-stream", action="store_true", help="skip the streamed C++ stage pipeline leg")
-    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch (96 geometries) leg")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch (96 geometries) and differentiable-FBP legs")
     ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
     ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
@@ -322,6 +323,55 @@ def main():
         except Exception as exc:  # reported, not fatal
             log(f"uneven batch skipped: {exc}")
 
+    # Differentiable FBP (16 slices, 512 detectors x 720 angles -> 512^2):
+    # forward = filter + backprojection; backward = ray-driven adjoint
+    # projector, filter-tap gradient and conjugate filter, per slice as the
+    # autograd function runs them.
+    adjoint = None
+    if not args.no_uneven:
+        try:
+            ga = g.replace(numberOfParallelDetectors=512, numberOfParallelProjections=720, numberOfPixels=512,
+                           filterType=0, fusedMasking=1)
+            S4, P4, D4, N4 = 16, 720, 512, 512
+            filt4, bp4 = device.Filter(ga, 0), device.Backprojection(ga)
+            x4 = torch.randn((S4, P4, D4), device=dev) * 0.01
+            y4 = x4.clone()
+            im4 = torch.empty((S4, N4, N4), device=dev)
+            gs4 = torch.empty_like(x4)
+            dt4 = torch.empty(D4 // 2 + 1, device=dev)
+
+            def fwd():
+                y4.copy_(x4)
+                filt4(y4, S4)
+                for s_ in range(S4):
+                    bp4(y4[s_], im4[s_])
+
+            def bwd():
+                for s_ in range(S4):
+                    bp4.adjoint(im4[s_], gs4[s_])
+                filt4.grad_taps(x4, gs4, S4, dt4)
+                filt4.adjoint(gs4, S4)
+
+            times = {}
+            with torch.cuda.stream(stream):
+                for name, fn in (("forward", fwd), ("backward", bwd)):
+                    fn()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(5):
+                        fn()
+                    e1.record(stream)
+                    stream.synchronize()
+                    times[name] = e0.elapsed_time(e1) / 5
+            adjoint = {"slices": S4, "forward_ms": round(times["forward"], 3), "backward_ms": round(times["backward"], 3),
+                       "forward_us_per_slice": round(times["forward"] * 1e3 / S4, 1),
+                       "backward_us_per_slice": round(times["backward"] * 1e3 / S4, 1),
+                       "what": "C4 geometry; backward = bp_adjoint_kernel per slice + fft_grad_taps_kernel + "
+                               "conjugate filter; one step of the autograd function"}
+            log(f"differentiable FBP: {adjoint}")
+        except Exception as exc:  # reported, not fatal
+            log(f"differentiable FBP timing skipped: {exc}")
+
     # PCIe-inclusive rate: the C++ stage pipeline (H2D -> ... -> D2H) streaming
     # pinned host frames; reported beside the device-resident value, never as it.
     streamed = None
@@ -417,6 +467,7 @@ def main():
                    "frac": round(bp_gbs / HBM_PEAK_GBS, 5), "bytes_per_launch": bp_bytes},
         "bp_f16": bp16,
         "uneven_batch": uneven,
+        "differentiable_fbp": adjoint,
         "streamed_pipeline": streamed,
         "cpu_baseline": cpu,
     }

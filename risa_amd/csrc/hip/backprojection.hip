@@ -696,46 +696,92 @@ __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restr
 }
 
 // Ray-driven projector: the exact transpose of bp_linear_kernel.  Thread per
-// (angle, detector k); walks the pixels whose floor(t) + cI is k or k - 1.
-// Ray-driven transpose of the linear backprojector.  Each pixel's t is
-// formed exactly as the forward kernel forms it for the tile (TX x TY) the
-// pixel falls in -- relative to that tile's integer origin T -- so the weights
-// are the forward's own and <BP s, x> = <s, BP^T x> holds to accumulation
-// rounding.
-__global__ void bp_adjoint_kernel(const float* __restrict__ img, float* __restrict__ sino,
-                                  const float* __restrict__ cosT, const float* __restrict__ sinT, BpParams p,
-                                  int TX, int TY)
+// (angle, detector k); walks the pixels whose floor(t) + cI can be k or k - 1.
+// Each pixel's t is formed exactly as the forward kernel forms it for the
+// tile (TX x TY) the pixel falls in -- relative to that tile's integer origin
+// T -- so the weights are the forward's own and <BP s, x> = <s, BP^T x> holds
+// to accumulation rounding.  A workgroup serves one angle: it first tabulates
+// every tile's origin for that angle in LDS (ltx, lty: log2 of the tile edge).
+// Dynamic LDS: tiles x 8 B + N x 12 B.
+__global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict__ img, float* __restrict__ sino,
+                                                         const float* __restrict__ cosT,
+                                                         const float* __restrict__ sinT, BpParams p, int ltx, int lty,
+                                                         int tilesY)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    int2* inside = reinterpret_cast<int2*>(smem);                     // [N] per row: columns [lo, hi] inside the mask
+    float2* origin = reinterpret_cast<float2*>(inside + p.N);         // per tile: (-T, bits of (int)T + cI)
+    float* coord = reinterpret_cast<float*>(origin + p.tilesX * tilesY);   // [N] (i - center) * scale
     const int a = blockIdx.y;
-    if (k >= p.D) return;
     const float c = cosT[a], s = sinT[a];
+    const int TX = 1 << ltx, TY = 1 << lty;
+    for (int t = threadIdx.x; t < p.tilesX * tilesY; t += blockDim.x) {
+        const int x0 = (t % p.tilesX) * TX, y0 = (t / p.tilesX) * TY;
+        const AngleWin aw = angle_window(c, s, ((float)x0 - p.center) * p.scale,
+                                         ((float)(x0 + TX - 1) - p.center) * p.scale,
+                                         ((float)y0 - p.center) * p.scale,
+                                         ((float)(y0 + TY - 1) - p.center) * p.scale, p.cI);
+        origin[t] = make_float2(aw.negT, __int_as_float((int)(-aw.negT) + p.cI));
+    }
+    for (int i = threadIdx.x; i < p.N; i += blockDim.x) {
+        coord[i] = ((float)i - p.center) * p.scale;
+        // the inside of row i is one interval around the centre (the test is
+        // monotone in |x - center|): found by bisection on the forward's own test
+        int lo = 0, hi = p.N - 1;
+        if (p.mask) {
+            const int mid = (p.N - 1) / 2;
+            if (bp_outside(mid, i, p.N, p.center)) {
+                lo = 1;
+                hi = 0;   // empty
+            } else {
+                int l = 0, r = mid;   // first inside column in [0, mid]
+                while (l < r) {
+                    const int m = (l + r) / 2;
+                    if (bp_outside(m, i, p.N, p.center)) l = m + 1;
+                    else r = m;
+                }
+                lo = l;
+                l = mid;
+                r = p.N - 1;   // last inside column in [mid, N - 1]
+                while (l < r) {
+                    const int m = (l + r + 1) / 2;
+                    if (bp_outside(m, i, p.N, p.center)) r = m - 1;
+                    else l = m;
+                }
+                hi = l;
+            }
+        }
+        inside[i] = make_int2(lo, hi);
+    }
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.D) return;
     const float lo = (float)(k - p.cI - 1), hi = (float)(k - p.cI + 1);   // t in [lo, hi)
     float acc = 0.0f;
     const bool alongX = fabsf(c) >= fabsf(s);
+    const float cc = alongX ? c : s, ss = alongX ? s : c;
     for (int u = 0; u < p.N; u++) {
         // u walks rows (alongX) or columns; solve the other coordinate's range
-        const float up = ((float)u - p.center) * p.scale;
-        const float cc = alongX ? c : s, ss = alongX ? s : c;
+        const float up = coord[u];
         const float v1 = (lo - up * ss) / cc, v2 = (hi - up * ss) / cc;
         const float vmin = fminf(v1, v2) / p.scale + p.center, vmax = fmaxf(v1, v2) / p.scale + p.center;
-        const int w0 = max(0, (int)floorf(vmin) - 1), w1 = min(p.N - 1, (int)ceilf(vmax) + 1);
+        // the forward's tile-relative t differs from this absolute solve by a
+        // few ulps of t; 1/64 pixel of slack covers that with room to spare
+        const int w0 = max(0, (int)floorf(vmin - 0.015625f)), w1 = min(p.N - 1, (int)ceilf(vmax + 0.015625f));
+        const int2 in_u = inside[u];
         for (int w = w0; w <= w1; w++) {
             const int x = alongX ? w : u, y = alongX ? u : w;
-            if (p.mask && bp_outside(x, y, p.N, p.center)) continue;
-            const float xp = ((float)x - p.center) * p.scale, yp = ((float)y - p.center) * p.scale;
-            const int x0 = (x / TX) * TX, y0 = (y / TY) * TY;
-            const AngleWin aw = angle_window(c, s, ((float)x0 - p.center) * p.scale,
-                                             ((float)(x0 + TX - 1) - p.center) * p.scale,
-                                             ((float)y0 - p.center) * p.scale,
-                                             ((float)(y0 + TY - 1) - p.center) * p.scale, p.cI);
-            const float tl = __builtin_fmaf(yp, s, __builtin_fmaf(xp, c, aw.negT));
-            int kk = floor_to_int(tl);
-            const float f = tl - (float)kk;
-            kk += (int)(-aw.negT) + p.cI;
-            const float v = img[(long)y * p.N + x];
-            if (kk == k) acc += (1.0f - f) * v;
-            else if (kk + 1 == k) acc += f * v;
+            const float2 o = origin[(y >> lty) * p.tilesX + (x >> ltx)];
+            const float tl = __builtin_fmaf(coord[y], s, __builtin_fmaf(coord[x], c, o.x));
+            const float fl = floorf(tl);
+            const int kk = (int)fl + __float_as_int(o.y);
+            const float f = tl - fl;
+            // weight of this pixel in sample k: 1 - f (k = kk), f (k = kk + 1), else 0;
+            // pixels outside the mask (a row interval, u or w) contribute nothing
+            const bool in = alongX ? (w >= in_u.x && w <= in_u.y) : (u >= inside[w].x && u <= inside[w].y);
+            float wt = kk == k ? 1.0f - f : (kk + 1 == k ? f : 0.0f);
+            wt = in ? wt : 0.0f;
+            if (wt != 0.0f) acc = __builtin_fmaf(wt, img[(long)y * p.N + x], acc);
         }
     }
     sino[(long)a * p.D + k] = acc * p.norm * p.out_scale;
@@ -1028,9 +1074,17 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
         // the transpose of the FP32 linear kernel the handle runs: same tile shape
         const BpVariant& v = kVariants[h->variant];
         BpParams p = make_params(h, v.tx, v.ty);
-        dim3 grid(ceil_div(p.D, 64), p.P);
-        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64), 0, as_stream(stream), img, sino, h->cosT.ptr,
-                           h->sinT.ptr, p, v.tx, v.ty);
+        const int tilesY = ceil_div(p.N, v.ty);
+        int ltx = 0, lty = 0;
+        while ((1 << ltx) < v.tx) ltx++;
+        while ((1 << lty) < v.ty) lty++;
+        if ((1 << ltx) != v.tx || (1 << lty) != v.ty)
+            fail(RISA_ERR_UNSUPPORTED, "adjoint needs power-of-two tile edges");
+        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 12;
+        if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
+        dim3 grid(ceil_div(p.D, 256), p.P);
+        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), lds, as_stream(stream), img, sino, h->cosT.ptr,
+                           h->sinT.ptr, p, ltx, lty, tilesY);
         check_launch("bp_adjoint_kernel");
     });
 }

@@ -329,15 +329,25 @@ __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restr
         for (int k = t0; k < K; k += T) partial[(long)blockIdx.x * K + k] = dst[k].x + dst[k + 2 * D].x;
 }
 
-__global__ void reduce_taps_kernel(const float* __restrict__ partial, int nblocks, int K, int D,
-                                   float* __restrict__ dtaps)
+// One workgroup per tap k: thread t sums partials t, t + 256, ... in order,
+// then a fixed LDS tree -- deterministic, and no thread walks all partials.
+__global__ void __launch_bounds__(256) reduce_taps_kernel(const float* __restrict__ partial, int nblocks, int K,
+                                                          int D, float* __restrict__ dtaps)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+    __shared__ float sum[256];
+    const int k = blockIdx.x;
     float acc = 0.0f;
-    for (int b = 0; b < nblocks; b++) acc += partial[(long)b * K + k];
-    const float wk = (k == 0 || (D % 2 == 0 && k == D / 2)) ? 1.0f : 2.0f;
-    dtaps[k] = wk * acc;
+    for (int b = threadIdx.x; b < nblocks; b += 256) acc += partial[(long)b * K + k];
+    sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sum[threadIdx.x] += sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float wk = (k == 0 || (D % 2 == 0 && k == D / 2)) ? 1.0f : 2.0f;
+        dtaps[k] = wk * sum[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -749,8 +759,8 @@ extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const 
         hipLaunchKernelGGL(fft_grad_taps_kernel, dim3(nblocks), dim3(256), lds, as_stream(stream), x, g, h->tw.ptr,
                            h->plan, rows, h->scratch.ptr);
         check_launch("fft_grad_taps_kernel");
-        hipLaunchKernelGGL(reduce_taps_kernel, dim3(ceil_div(K, 256)), dim3(256), 0, as_stream(stream), h->scratch.ptr,
-                           nblocks, K, D, dtaps);
+        hipLaunchKernelGGL(reduce_taps_kernel, dim3(K), dim3(256), 0, as_stream(stream), h->scratch.ptr, nblocks, K, D,
+                           dtaps);
         check_launch("reduce_taps_kernel");
     });
 }
