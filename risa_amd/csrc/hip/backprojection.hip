@@ -68,10 +68,11 @@ __device__ __forceinline__ float bp_t(float xp, float yp, float c, float s)
 }
 
 // Masking.cu:162-166: outside when the float distance exceeds N*N*0.25 (double).
+// Rounded products and sum, no contraction: symmetric in x and y.
 __device__ __forceinline__ bool bp_outside(int x, int y, int N, float center)
 {
     const float dX = (float)x - center, dY = (float)y - center;
-    const float dist = dX * dX + dY * dY;
+    const float dist = __fadd_rn(__fmul_rn(dX, dX), __fmul_rn(dY, dY));
     return (double)dist > (double)N * N * 0.25;
 }
 
@@ -695,24 +696,107 @@ __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restr
     img[(long)y * p.N + x] = sum;
 }
 
-// Ray-driven projector: the exact transpose of bp_linear_kernel.  Thread per
-// (angle, detector k); walks the pixels whose floor(t) + cI can be k or k - 1.
-// Each pixel's t is formed exactly as the forward kernel forms it for the
-// tile (TX x TY) the pixel falls in -- relative to that tile's integer origin
-// T -- so the weights are the forward's own and <BP s, x> = <s, BP^T x> holds
-// to accumulation rounding.  A workgroup serves one angle: it first tabulates
-// every tile's origin for that angle in LDS (ltx, lty: log2 of the tile edge).
-// Dynamic LDS: tiles x 8 B + N x 12 B.
-__global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict__ img, float* __restrict__ sino,
+// out[x][y] = in[y][x] for an N x N image, 32 x 32 tiles through LDS.
+__global__ void __launch_bounds__(256) transpose_kernel(const float* __restrict__ in, float* __restrict__ out, int N)
+{
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int y = by + r, x = bx + threadIdx.x;
+        if (y < N && x < N) tile[r][threadIdx.x] = in[(long)y * N + x];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int x = bx + r, y = by + threadIdx.x;
+        if (y < N && x < N) out[(long)x * N + y] = tile[threadIdx.x][r];
+    }
+}
+
+// Ray-driven projector: the exact transpose of bp_linear_kernel.
+//
+// A workgroup serves one angle and kAdjK consecutive detectors, one per lane;
+// its four waves split the image lines (wave w takes lines w, w + 4, ...) and
+// their partial sums are added in wave order, so the result does not depend
+// on scheduling.  Each pixel's t is formed exactly as the forward kernel
+// forms it for the tile (TX x TY) the pixel falls in -- relative to that
+// tile's integer origin T, tabulated per angle in LDS -- so the weights are
+// the forward's own and <BP s, x> = <s, BP^T x> holds to accumulation
+// rounding: pixel (x, y) with k = floor(t) + T + cI adds (1 - f) img to
+// entry k and f img to entry k + 1; pixels outside the mask add nothing.
+// Detector k gathers the pixels with t in [k - cI - 1, k - cI + 1): on each
+// line, a run of about 2 / (|cos| scale) pixels, found from the line's
+// coordinate with 1/64 pixel of slack (the few ulps between that solve and
+// the tile-relative t).  Lines run along x when |cos| >= |sin|, else along y,
+// read as rows of the transposed image imgT so a wave's loads stay close
+// together.  ltx, lty: log2 tile edges.
+constexpr int kAdjK = 64;
+
+// One lane's sum over its detector k, wave walking lines wave, wave + 4, ...
+// Per line the wave first evaluates every pixel of the span its 64 detectors
+// reach, one pixel per lane (coalesced loads, each pixel once): k = floor(t) +
+// T + cI and its two contributions (1 - f) v, f v go to the wave's LDS buffer
+// (cap entries).  Each lane then adds up the contributions to its detector
+// along its own run of the span.  AX: lines run along x (rows of img); else
+// along y (rows of imgT).  [SpLo, SpHi]: the span's bounds before the line's
+// term, [Alo, Ahi]: the lane's run's.
+template <bool AX>
+__device__ __forceinline__ float adj_gather(const BpParams& p, const int2* inside, const float2* origin,
+                                            const float* __restrict__ src, float c, float s, int k, float B,
+                                            float Alo, float Ahi, float SpLo, float SpHi, int ltx, int lty, int wave,
+                                            int lane, int* bk, float2* bv, int cap)
+{
+    float acc = 0.0f;
+    for (int u = wave; u < p.N; u += 4) {
+        const float lc = ((float)u - p.center) * p.scale;
+        const int2 in_u = inside[u];
+        const int W0 = max(in_u.x, (int)floorf(__builtin_fmaf(-B, lc, SpLo)));
+        const int W1 = min(min(in_u.y, (int)ceilf(__builtin_fmaf(-B, lc, SpHi))), W0 + cap - 1);
+        if (W0 > W1) continue;   // the same for the whole wave
+        const int trow = AX ? (u >> lty) * p.tilesX : (u >> ltx);
+        const float* line = src + (long)u * p.N;
+        for (int w = W0 + lane; w <= W1; w += 64) {
+            const float2 o = origin[AX ? trow + (w >> ltx) : (w >> lty) * p.tilesX + trow];
+            const float v = line[w];
+            const float wp = ((float)w - p.center) * p.scale;
+            const float tl = AX ? __builtin_fmaf(lc, s, __builtin_fmaf(wp, c, o.x))
+                                : __builtin_fmaf(wp, s, __builtin_fmaf(lc, c, o.x));
+            const float fl = floorf(tl);
+            const float f = tl - fl;
+            bk[w - W0] = (int)fl + __float_as_int(o.y);
+            bv[w - W0] = make_float2((1.0f - f) * v, f * v);
+        }
+        // the buffer is the wave's own: LDS completes a wave's accesses in order
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int r0 = max(W0, (int)floorf(__builtin_fmaf(-B, lc, Alo)));
+        const int r1 = min(W1, (int)ceilf(__builtin_fmaf(-B, lc, Ahi)));
+        for (int w = r0; w <= r1; w++) {
+            const int kk = bk[w - W0];
+            const float2 e = bv[w - W0];
+            acc += kk == k ? e.x : (kk + 1 == k ? e.y : 0.0f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict__ img,
+                                                         const float* __restrict__ imgT, float* __restrict__ sino,
                                                          const float* __restrict__ cosT,
                                                          const float* __restrict__ sinT, BpParams p, int ltx, int lty,
-                                                         int tilesY)
+                                                         int tilesY, int cap)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    int2* inside = reinterpret_cast<int2*>(smem);                     // [N] per row: columns [lo, hi] inside the mask
+    int2* inside = reinterpret_cast<int2*>(smem);                     // [N] per line: [lo, hi] inside the mask
     float2* origin = reinterpret_cast<float2*>(inside + p.N);         // per tile: (-T, bits of (int)T + cI)
-    float* coord = reinterpret_cast<float*>(origin + p.tilesX * tilesY);   // [N] (i - center) * scale
-    const int a = blockIdx.y;
+    float* part = reinterpret_cast<float*>(origin + p.tilesX * tilesY);   // [4 waves][kAdjK]
+    int* bk = reinterpret_cast<int*>(part + 4 * kAdjK);                     // [4 waves][cap]
+    float2* bv = reinterpret_cast<float2*>(bk + 4 * cap);                   // [4 waves][cap]
+    const int a = blockIdx.y, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int k = blockIdx.x * kAdjK + lane;
     const float c = cosT[a], s = sinT[a];
     const int TX = 1 << ltx, TY = 1 << lty;
     for (int t = threadIdx.x; t < p.tilesX * tilesY; t += blockDim.x) {
@@ -724,9 +808,9 @@ __global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict
         origin[t] = make_float2(aw.negT, __int_as_float((int)(-aw.negT) + p.cI));
     }
     for (int i = threadIdx.x; i < p.N; i += blockDim.x) {
-        coord[i] = ((float)i - p.center) * p.scale;
-        // the inside of row i is one interval around the centre (the test is
-        // monotone in |x - center|): found by bisection on the forward's own test
+        // the inside of line i is one interval around the centre (the test is
+        // monotone in |x - center| and symmetric in x and y, so rows and
+        // columns share it): found by bisection on the forward's own test
         int lo = 0, hi = p.N - 1;
         if (p.mask) {
             const int mid = (p.N - 1) / 2;
@@ -754,37 +838,30 @@ __global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict
         inside[i] = make_int2(lo, hi);
     }
     __syncthreads();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= p.D) return;
-    const float lo = (float)(k - p.cI - 1), hi = (float)(k - p.cI + 1);   // t in [lo, hi)
-    float acc = 0.0f;
     const bool alongX = fabsf(c) >= fabsf(s);
+    // on line u (coordinate up), t = w' cc + up ss with w' = (w - center) scale:
+    // w = (t - up ss) / (cc scale) + center
     const float cc = alongX ? c : s, ss = alongX ? s : c;
-    for (int u = 0; u < p.N; u++) {
-        // u walks rows (alongX) or columns; solve the other coordinate's range
-        const float up = coord[u];
-        const float v1 = (lo - up * ss) / cc, v2 = (hi - up * ss) / cc;
-        const float vmin = fminf(v1, v2) / p.scale + p.center, vmax = fmaxf(v1, v2) / p.scale + p.center;
-        // the forward's tile-relative t differs from this absolute solve by a
-        // few ulps of t; 1/64 pixel of slack covers that with room to spare
-        const int w0 = max(0, (int)floorf(vmin - 0.015625f)), w1 = min(p.N - 1, (int)ceilf(vmax + 0.015625f));
-        const int2 in_u = inside[u];
-        for (int w = w0; w <= w1; w++) {
-            const int x = alongX ? w : u, y = alongX ? u : w;
-            const float2 o = origin[(y >> lty) * p.tilesX + (x >> ltx)];
-            const float tl = __builtin_fmaf(coord[y], s, __builtin_fmaf(coord[x], c, o.x));
-            const float fl = floorf(tl);
-            const int kk = (int)fl + __float_as_int(o.y);
-            const float f = tl - fl;
-            // weight of this pixel in sample k: 1 - f (k = kk), f (k = kk + 1), else 0;
-            // pixels outside the mask (a row interval, u or w) contribute nothing
-            const bool in = alongX ? (w >= in_u.x && w <= in_u.y) : (u >= inside[w].x && u <= inside[w].y);
-            float wt = kk == k ? 1.0f - f : (kk + 1 == k ? f : 0.0f);
-            wt = in ? wt : 0.0f;
-            if (wt != 0.0f) acc = __builtin_fmaf(wt, img[(long)y * p.N + x], acc);
-        }
+    const float r = 1.0f / (cc * p.scale), B = ss * r;
+    const float A0 = (float)(k - p.cI - 1) * r + p.center, A1 = (float)(k - p.cI + 1) * r + p.center;
+    const float Alo = fminf(A0, A1) - 0.015625f, Ahi = fmaxf(A0, A1) + 0.015625f;
+    // the span of the workgroup's detectors k0 .. k0 + kAdjK - 1
+    const int k0 = blockIdx.x * kAdjK;
+    const float S0 = (float)(k0 - p.cI - 1) * r + p.center, S1 = (float)(k0 + kAdjK - p.cI) * r + p.center;
+    const float SpLo = fminf(S0, S1) - 0.015625f, SpHi = fmaxf(S0, S1) + 0.015625f;
+    int* wbk = bk + wave * cap;
+    float2* wbv = bv + wave * cap;
+    const float acc =
+        alongX ? adj_gather<true>(p, inside, origin, img, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk,
+                                  wbv, cap)
+               : adj_gather<false>(p, inside, origin, imgT, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk,
+                                   wbv, cap);
+    part[wave * kAdjK + lane] = acc;
+    __syncthreads();
+    if (wave == 0 && k < p.D) {
+        const float sum = ((part[lane] + part[kAdjK + lane]) + part[2 * kAdjK + lane]) + part[3 * kAdjK + lane];
+        sino[(long)a * p.D + k] = sum * p.norm * p.out_scale;
     }
-    sino[(long)a * p.D + k] = acc * p.norm * p.out_scale;
 }
 
 // Standalone Masking (Masking.cu:113-144): optional min-max normalisation
@@ -909,6 +986,7 @@ struct risa_bp_s {
     int variant_f16 = 0;
     bool fuse_mask = false;
     risa::DeviceBuffer<uint2> x4;   // staging for risa_hip_bp_run_f16's plain layout
+    risa::DeviceBuffer<float> imgT;  // transposed image for risa_hip_bp_adjoint_run
 };
 
 using namespace risa;
@@ -1080,11 +1158,25 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
         while ((1 << lty) < v.ty) lty++;
         if ((1 << ltx) != v.tx || (1 << lty) != v.ty)
             fail(RISA_ERR_UNSUPPORTED, "adjoint needs power-of-two tile edges");
-        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 12;
+        // a line's span for kAdjK detectors: (kAdjK + 2) / (|cos or sin| scale)
+        // pixels with |cos or sin| >= 1/sqrt(2), plus slack and rounding
+        const int cap = std::min(p.N, (int)std::ceil((kAdjK + 2) * 1.41422 / p.scale) + 6);
+        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + 4 * kAdjK * sizeof(float) +
+                           4 * (size_t)cap * 12;
         if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
-        dim3 grid(ceil_div(p.D, 256), p.P);
-        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), lds, as_stream(stream), img, sino, h->cosT.ptr,
-                           h->sinT.ptr, p, ltx, lty, tilesY);
+        DeviceGuard dg(h->device);
+        hipStream_t s = as_stream(stream);
+        const size_t need = (size_t)p.N * p.N;
+        if (h->imgT.count < need) {
+            RISA_HIP_CHECK(hipStreamSynchronize(s));
+            h->imgT.resize(need);
+        }
+        hipLaunchKernelGGL(transpose_kernel, dim3(ceil_div(p.N, 32), ceil_div(p.N, 32)), dim3(32, 8), 0, s, img,
+                           h->imgT.ptr, p.N);
+        check_launch("transpose_kernel");
+        dim3 grid(ceil_div(p.D, kAdjK), p.P);
+        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), lds, s, img, h->imgT.ptr, sino, h->cosT.ptr,
+                           h->sinT.ptr, p, ltx, lty, tilesY, cap);
         check_launch("bp_adjoint_kernel");
     });
 }
