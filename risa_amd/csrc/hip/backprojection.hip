@@ -67,12 +67,21 @@ __device__ __forceinline__ float bp_t(float xp, float yp, float c, float s)
     return __builtin_fmaf(yp, s, __fmul_rn(xp, c));
 }
 
+// A rounded product that multiply-add contraction cannot fuse into a
+// following sum (under -ffp-contract=fast even __fmul_rn / __fadd_rn fuse:
+// in HIP they are plain operators).  Only the sign of a zero product can
+// differ from x * y, which no caller here observes.
+__device__ __forceinline__ float rounded_mul(float x, float y)
+{
+    return __builtin_fmaf(x, y, 0.0f);
+}
+
 // Masking.cu:162-166: outside when the float distance exceeds N*N*0.25 (double).
-// Rounded products and sum, no contraction: symmetric in x and y.
+// Rounded products and sum: symmetric in x and y.
 __device__ __forceinline__ bool bp_outside(int x, int y, int N, float center)
 {
     const float dX = (float)x - center, dY = (float)y - center;
-    const float dist = __fadd_rn(__fmul_rn(dX, dX), __fmul_rn(dY, dY));
+    const float dist = rounded_mul(dX, dX) + rounded_mul(dY, dY);
     return (double)dist > (double)N * N * 0.25;
 }
 
@@ -675,10 +684,13 @@ __global__ void bp_nearest_kernel(const float* __restrict__ sino, float* __restr
     if (x >= p.N || y >= p.N) return;
     float sum = 0.0f;
     if (tex) {
+        // backProjectTex: __fadd_rn(xp * cos, fma(yp, sin, cI)) with the sum and
+        // the product rounded separately, as CUDA's intrinsics guarantee
         const float cI = (float)(p.D * 0.5);
         const float xp = ((float)x - p.center) * p.scale, yp = ((float)y - p.center) * p.scale;
         for (int a = 0; a < p.P; a++) {
-            const int t = (int)__fadd_rn(__fmul_rn(xp, cosT[a]), __fmaf_rn(yp, sinT[a], cI));
+            const float u = rounded_mul(xp, cosT[a]), v = __builtin_fmaf(yp, sinT[a], cI);
+            const int t = (int)(u + v);
             if (t >= 0 && t < p.D) sum += sino[(long)a * p.D + t];
         }
         sum = sum * p.norm;
