@@ -72,6 +72,7 @@ def kernel_only(args, g, dev):
     agree with.  `copy` is a plain device copy of known size, the scale for
     the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
+    import risa_amd
     from risa_amd import device
 
     N, P, D = g.numberOfPixels, g.numberOfParallelProjections, g.numberOfParallelDetectors
@@ -96,6 +97,28 @@ def kernel_only(args, g, dev):
                 img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
                 fn, per = (lambda: bp.f16x4(sino, 256.0, B, img)), B
                 what = f"bp FP16 x4 layout, {B} slices {N}^2 x {P} per launch, variant {bp.variant >> 8}"
+        elif args.kernel in ("att", "att_mm", "reorder"):
+            # 32 C1 frames per launch: attenuation of detector-major frames,
+            # the same reading ROFEX module-major frames through the fused
+            # reordering, and the standalone reordering
+            from risa_amd import synth
+            F, Pf, Df = 32, g.numberOfFanProjections, g.numberOfFanDetectors
+            avg_ref, avg_dark = synth.prepare_reference(g, synth.reference_frames(g, 4, seed=901))
+            raw = torch.from_numpy(rng.integers(100, 60000, size=(F, Pf, Df), dtype=np.uint16)).to(dev)
+            planes = np.arange(F, dtype=np.int32) % g.numberOfPlanes
+            per = F
+            if args.kernel == "reorder":
+                out16 = torch.empty_like(raw)
+                fn = lambda: device.reorder(raw, F, Pf, Df, out16)
+                what = f"reorder module-major -> detector-major, {F} frames of {Pf} x {Df} uint16 per launch"
+            else:
+                att = device.Attenuation(g, avg_ref, avg_dark)
+                if args.kernel == "att_mm":
+                    att.input_layout = risa_amd.LAYOUT_MODULE_MAJOR
+                out = torch.empty((F, Pf, Df), dtype=torch.float32, device=dev)
+                fn = lambda: att.batch(raw, planes, out)
+                layout = "module-major (reordering fused)" if args.kernel == "att_mm" else "detector-major"
+                what = f"attenuation, {F} frames of {Pf} x {Df}, {layout} input"
         else:
             n = 64 << 20
             src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -130,7 +153,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch (96 geometries) and differentiable-FBP legs")
     ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
-    ap.add_argument("--kernel", choices=("bp", "bp16", "copy"),
+    ap.add_argument("--kernel", choices=("bp", "bp16", "att", "att_mm", "reorder", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     args = ap.parse_args()
@@ -138,7 +161,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(frames shard round-robin with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(frames shard round-robin with no exchange). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

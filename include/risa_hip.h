@@ -53,6 +53,13 @@ enum risa_filter_mode {
 
 enum risa_dtype { RISA_F32 = 0, RISA_F16 = 1 };
 
+/* Layout of raw uint16 fan frames.  DETECTOR_MAJOR: [P_f][D_f], what the
+ * reference's Attenuation receives.  MODULE_MAJOR: the ROFEX frame as it
+ * arrives (OfflineLoader.cu:56-78, ReceiverModule.cpp:56-116), 27 modules x
+ * P_f projections x 16 detectors, which the reference's Reordering stage
+ * permutes (Reordering.cu:180-204). */
+enum risa_input_layout { RISA_LAYOUT_DETECTOR_MAJOR = 0, RISA_LAYOUT_MODULE_MAJOR = 1 };
+
 /* The configuration keys the hot path reads (RISA/example/config.cfg), as
  * libconfig holds them.  numberOfFanProjections is the derived
  * samplingRate*1000000/scanRate (Attenuation.cu:409, Fan2Para.cu:610).
@@ -163,6 +170,22 @@ int risa_hip_attenuation_run(risa_attenuation_t h, const uint16_t* raw, int plan
 int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16_t* raw,
                                    const int32_t* planes, int frames, float* out, void* stream);
 int risa_hip_attenuation_destroy(risa_attenuation_t h);
+/* Input layout of the raw frames the handle reads (default DETECTOR_MAJOR).
+ * MODULE_MAJOR fuses the Reordering stage into the attenuation load: the
+ * result equals Reordering (Reordering.cu:127-150) followed by Attenuation,
+ * without the intermediate frame.  Needs D_f >= 432 (RISA_ERR_UNSUPPORTED
+ * otherwise: the reference reads past the frame there). */
+int risa_hip_attenuation_set_input_layout(risa_attenuation_t h, int layout);
+int risa_hip_attenuation_get_input_layout(risa_attenuation_t h, int* layout);
+
+/* ------------------------------------------------------------- Reordering -- */
+/* Replaces Reordering::processor's reorder kernel with its createHashTable
+ * permutation (Reordering.cu:127-150, 180-204): `frames` consecutive
+ * module-major frames raw -> detector-major ordered, each [P_f][D_f] uint16,
+ * ordered[i] = raw[hash(i)] with the reference's table (entries past
+ * P_f*432 read sample 0).  D_f >= 432; not in place; runs on the current
+ * device, stream-ordered. */
+int risa_hip_reorder_run(const uint16_t* raw, int frames, int Pf, int Df, uint16_t* ordered, void* stream);
 
 /* --------------------------------------------------------------- Fan2Para -- */
 /* Interpolation-weight precision of the rebinning table. */
@@ -314,6 +337,11 @@ int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
  * RISA_ERR_UNSUPPORTED.  No reference counterpart. */
 int risa_hip_chain_set_variant(risa_chain_t h, int variant);
 int risa_hip_chain_get_variant(risa_chain_t h, int* variant);
+/* Layout of the raw frames risa_hip_chain_run reads (risa_input_layout;
+ * MODULE_MAJOR = the reference's H2D -> Reordering -> Attenuation, with the
+ * reordering fused into the attenuation load).  Only before the chain's first
+ * graph run, else RISA_ERR_UNSUPPORTED. */
+int risa_hip_chain_set_input_layout(risa_chain_t h, int layout);
 int risa_hip_chain_destroy(risa_chain_t h);
 
 #ifdef __cplusplus

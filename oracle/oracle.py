@@ -132,6 +132,17 @@ def reference_average(g, ref_frames, R):
     return avg_ref, avg_dark, ref
 
 
+def reorder_u16(raw, P, D):
+    """Reordering (Reordering.cu:180-204): module-major frames [frames][P*D]
+    -> detector-major [frames][P][D]."""
+    raw = _c(raw, np.uint16).reshape(-1, P * D)
+    out = np.empty((raw.shape[0], P, D), np.uint16)
+    rc = lib().orc_reorder_u16(C.c_int(P), C.c_int(D), C.c_int(raw.shape[0]), _p(raw), _p(out))
+    if rc:
+        raise ValueError("the reference's reordering table indexes past frames with fewer than 432 detectors")
+    return out
+
+
 def attenuation_f32(g, raw, plane, avg_ref, avg_dark, mask):
     og = geom(g)
     out = np.empty((og.numberOfFanProjections, og.numberOfFanDetectors), np.float32)

@@ -203,3 +203,25 @@ void orc_attenuation_f64(const orc_geom* g, const uint16_t* raw, int plane,
             out[sinoIndex] = -log(numerator / denominator) * (double)mask[sinoIndex];
         }
 }
+
+/* Reordering::createHashTable (Reordering.cu:180-192) and the reorder kernel
+ * (:194-204) for `frames` consecutive frames: the table has P*D entries,
+ * zero-initialised, and its first P*27*16 are filled in (projection, module,
+ * detector) loop order; ordered[i] = raw[table[i]].  Returns -1 when D < 432,
+ * where the reference's table indexes past the frame. */
+int orc_reorder_u16(int P, int D, int frames, const uint16_t* raw, uint16_t* ordered)
+{
+    const int modules = 27, perModule = 16;
+    if (D < modules * perModule) return -1;
+    const size_t n = (size_t)P * D;
+    int* table = (int*)calloc(n, sizeof(int));
+    size_t i = 0;
+    for (int projInd = 0; projInd < P; projInd++)
+        for (int modInd = 0; modInd < modules; modInd++)
+            for (int detInd = 0; detInd < perModule; detInd++)
+                table[i++] = detInd + projInd * perModule + modInd * perModule * P;
+    for (int f = 0; f < frames; f++)
+        for (size_t e = 0; e < n; e++) ordered[f * n + e] = raw[f * n + (size_t)table[e]];
+    free(table);
+    return 0;
+}

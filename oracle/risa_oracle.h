@@ -93,6 +93,9 @@ void orc_reference_average(const orc_geom* g, uint16_t* ref, int R,
 
 /* ---------------------------------------------------------------- a1 --- */
 /* computeAttenuation, Attenuation.cu:416-444 with temp = pow(10,-5) (:144). */
+/* Reordering (Reordering.cu:180-204): module-major -> detector-major. */
+int orc_reorder_u16(int P, int D, int frames, const uint16_t* raw, uint16_t* ordered);
+
 void orc_attenuation_f32(const orc_geom* g, const uint16_t* raw, int plane,
                          const float* avg_ref, const float* avg_dark,
                          const float* mask, float* out);

@@ -11,13 +11,14 @@ from __future__ import annotations
 from pathlib import Path
 
 from ._lib import (CHAIN_MIXED, CHAIN_PLAIN, F16, F32, FILTER_MODE_FFT32, FILTER_MODE_GEMM_SPLIT,
-                   FILTER_TYPES, BpJob, Geom, RisaError, check, load, ptr)
+                   FILTER_TYPES, LAYOUT_DETECTOR_MAJOR, LAYOUT_MODULE_MAJOR, BpJob, Geom, RisaError, check, load,
+                   ptr)
 
 CONFIG_DIR = Path(__file__).resolve().parent / "configs"
 
 __all__ = ["Geom", "BpJob", "RisaError", "load", "check", "ptr", "geom_from_config", "config_path",
            "FILTER_TYPES", "FILTER_MODE_FFT32", "FILTER_MODE_GEMM_SPLIT", "CHAIN_PLAIN", "CHAIN_MIXED",
-           "F32", "F16"]
+           "F32", "F16", "LAYOUT_DETECTOR_MAJOR", "LAYOUT_MODULE_MAJOR"]
 
 
 def config_path(name: str) -> Path:

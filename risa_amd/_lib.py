@@ -28,6 +28,8 @@ CHAIN_PLAIN = 0
 CHAIN_MIXED = 1
 F32 = 0
 F16 = 1
+LAYOUT_DETECTOR_MAJOR = 0
+LAYOUT_MODULE_MAJOR = 1
 
 
 class Geom(C.Structure):
@@ -150,6 +152,9 @@ _SIGS = {
     "risa_hip_attenuation_run": [P, P, C.c_int, P, P],
     "risa_hip_attenuation_run_batch": [P, P, P, C.c_int, P, P],
     "risa_hip_attenuation_destroy": [P],
+    "risa_hip_attenuation_set_input_layout": [P, C.c_int],
+    "risa_hip_attenuation_get_input_layout": [P, C.POINTER(C.c_int)],
+    "risa_hip_reorder_run": [P, C.c_int, C.c_int, C.c_int, P, P],
     "risa_hip_fan2para_create": [C.POINTER(Geom), C.c_int, C.POINTER(P)],
     "risa_hip_fan2para_create_ex": [C.POINTER(Geom), C.c_int, C.c_int, C.POINTER(P)],
     "risa_hip_fan2para_run": [P, P, C.c_int, P, P],
@@ -192,6 +197,7 @@ _SIGS = {
     "risa_hip_chain_intermediate": [P, C.c_int, C.POINTER(P)],
     "risa_hip_chain_set_variant": [P, C.c_int],
     "risa_hip_chain_get_variant": [P, C.POINTER(C.c_int)],
+    "risa_hip_chain_set_input_layout": [P, C.c_int],
     "risa_hip_chain_destroy": [P],
     # synthetic source (include/risa_synth.h)
     "risa_synth_phantom": [C.c_uint64, C.c_double, C.POINTER(SynthOpts), P, C.c_int],

@@ -8,7 +8,11 @@
 // recorded frames from RAM); the sink counts slices, measures latency and
 // keeps the first few slices for checking.
 //
-//   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking]
+//   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major]
+//
+// --module-major: the source holds ROFEX module-major frames (27 modules x
+// P_f x 16, D_f = 432) and a Reordering stage follows H2D, as
+// RISA/example/main.cpp:112-126 wires it.
 //
 // Prints one JSON line: frames, seconds, slices/s, mean and max latency.
 // --save writes K slices as records {int32 index, int32 frame, int32 plane,
@@ -44,10 +48,14 @@ class SyntheticLoader {
 public:
     using manager_type = glados::hip::HostMemoryManager<uint16_t>;
 
-    SyntheticLoader(const std::string& configFile, long frames, int unique, Provenance* prov)
+    SyntheticLoader(const std::string& configFile, long frames, int unique, Provenance* prov, bool module_major)
         : frames_(frames), prov_(prov)
     {
         g_ = risa::read_geometry(configFile);
+        const int Pf = g_.numberOfFanProjections, Df = g_.numberOfFanDetectors;
+        if (module_major && Df != 27 * 16)
+            throw std::runtime_error("--module-major needs 27 modules of 16 detectors (numberOfFanDetectors = 432)");
+        std::vector<uint16_t> frame;
         const int planes = g_.numberOfPlanes;
         const std::size_t n = (std::size_t)g_.numberOfFanDetectors * g_.numberOfFanProjections;
         auto* pool = glados::MemoryPool<manager_type>::instance();
@@ -64,6 +72,14 @@ public:
             risa::check(risa_synth_fan_frame(&g_, k, 1000ull + (uint64_t)u, 55ull + (uint64_t)(u * planes + k), &o,
                                              img.data()),
                         "risa_synth_fan_frame");
+            if (module_major) {
+                // [P_f][module * 16 + d] -> [module][P_f][16], what the detector sends
+                frame.assign(img.data(), img.data() + n);
+                for (int p = 0; p < Pf; p++)
+                    for (int m = 0; m < 27; m++)
+                        for (int d = 0; d < 16; d++)
+                            img.data()[((std::size_t)m * Pf + p) * 16 + d] = frame[(std::size_t)p * Df + m * 16 + d];
+            }
             ids_[img.data()] = {u, k};
             held.push_back(std::move(img));
         }
@@ -128,13 +144,15 @@ private:
 int main(int argc, char** argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking]\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major]\n",
+                     argv[0]);
         return 2;
     }
     const std::string config = argv[1];
     long frames = 200;
     int unique = 8, keep = 0;
-    bool masking = false;
+    bool masking = false, module_major = false;
     std::string save_path;
     for (int i = 2; i < argc; i++) {
         const std::string a = argv[i];
@@ -144,6 +162,7 @@ int main(int argc, char** argv)
             keep = std::atoi(argv[++i]);
             save_path = argv[++i];
         } else if (a == "--masking") masking = true;
+        else if (a == "--module-major") module_major = true;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -159,8 +178,9 @@ int main(int argc, char** argv)
     st.keep = keep;
     try {
         Pipeline pipeline;
-        auto source = pipeline.create<Source>(config, frames, unique, &prov);
+        auto source = pipeline.create<Source>(config, frames, unique, &prov, module_major);
         auto h2d = pipeline.create<Stage<risa::H2D>>(config);
+        auto reordering = module_major ? pipeline.create<Stage<risa::Reordering>>(config) : nullptr;
         auto attenuation = pipeline.create<Stage<risa::Attenuation>>(config);
         auto fan2para = pipeline.create<Stage<risa::Fan2Para>>(config);
         auto filter = pipeline.create<Stage<risa::Filter>>(config);
@@ -170,7 +190,12 @@ int main(int argc, char** argv)
         auto sink = pipeline.create<Sink>("", "IMG", &st);
 
         pipeline.connect(source, h2d);
-        pipeline.connect(h2d, attenuation);
+        if (reordering) {
+            pipeline.connect(h2d, reordering);
+            pipeline.connect(reordering, attenuation);
+        } else {
+            pipeline.connect(h2d, attenuation);
+        }
         pipeline.connect(attenuation, fan2para);
         pipeline.connect(fan2para, filter);
         pipeline.connect(filter, backprojection);
@@ -183,10 +208,11 @@ int main(int argc, char** argv)
         pipeline.connect(d2h, sink);
 
         const auto t0 = std::chrono::high_resolution_clock::now();
-        if (masking_stage)
-            pipeline.run(source, h2d, attenuation, fan2para, filter, backprojection, masking_stage, d2h, sink);
-        else
-            pipeline.run(source, h2d, attenuation, fan2para, filter, backprojection, d2h, sink);
+        pipeline.run(source, h2d);
+        if (reordering) pipeline.run(reordering);
+        pipeline.run(attenuation, fan2para, filter, backprojection);
+        if (masking_stage) pipeline.run(masking_stage);
+        pipeline.run(d2h, sink);
         pipeline.wait();
         const auto t1 = std::chrono::high_resolution_clock::now();
         const double secs = std::chrono::duration<double>(t1 - t0).count();

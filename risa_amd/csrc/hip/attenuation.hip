@@ -9,6 +9,11 @@
 //   refm = +max(ref - dark, eps) where mask = 1, -max(ref - dark, eps) where mask = 0
 // (the clamped denominator is always positive, so its sign bit is free).
 // A frame then streams 2 B raw + 4 B refm in and 4 B out per element.
+//
+// Input layouts: detector-major [P_f][D_f] (what Attenuation receives in the
+// reference, after its Reordering stage) or the ROFEX module-major raw frame,
+// which the kernels read through the Reordering permutation directly
+// (Reordering.cu:180-204, see reorder_source below).
 #include "device_common.h"
 
 #include <algorithm>
@@ -24,11 +29,58 @@ struct PlaneList {
 
 constexpr long kAttGrid = 256;   // workgroups per frame (grid-stride beyond)
 
+// Reordering.cu:180-204: ordered[i] = raw[hash[i]] over the frame's flat index
+// i = x + y * D_f, with the hash table built by createHashTable for 27
+// modules of 16 detectors: entry i = p * 432 + m * 16 + d (i < P_f * 432)
+// holds d + p * 16 + m * 16 * P_f; the rest of the zero-initialised table
+// stays 0.  For D_f = 432 this moves [module][projection][16] to
+// [projection][module * 16 + detector].  (D_f < 432 reads past the frame in
+// the reference and is refused by the host.)
+constexpr int kRofexModules = 27, kRofexModuleDetectors = 16;
+constexpr int kRofexDetectors = kRofexModules * kRofexModuleDetectors;
+__device__ __forceinline__ long reorder_source(long i, int Pf)
+{
+    if (i >= (long)Pf * kRofexDetectors) return 0;
+    const int p = (int)(i / kRofexDetectors), r = (int)(i - (long)p * kRofexDetectors);
+    const int m = r / kRofexModuleDetectors, d = r - m * kRofexModuleDetectors;
+    return d + (long)p * kRofexModuleDetectors + (long)m * kRofexModuleDetectors * Pf;
+}
+
+__global__ void __launch_bounds__(256) reorder_kernel(const uint16_t* __restrict__ raw, uint16_t* __restrict__ out,
+                                                      long n, int Pf)
+{
+    const uint16_t* r = raw + (long)blockIdx.y * n;
+    uint16_t* o = out + (long)blockIdx.y * n;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        o[i] = r[reorder_source(i, Pf)];
+}
+
+// Four samples per thread (n % 4 == 0, 8-byte aligned frames): four ordered
+// samples at a multiple of 4 come from four consecutive samples of one
+// module, or all from sample 0 past the table's filled part.
+__global__ void __launch_bounds__(256) reorder4_kernel(const uint16_t* __restrict__ raw, uint16_t* __restrict__ out,
+                                                       long n, int Pf)
+{
+    const uint16_t* r = raw + (long)blockIdx.y * n;
+    uint16_t* o = out + (long)blockIdx.y * n;
+    const long filled = (long)Pf * kRofexDetectors;
+    for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < n; i += (long)gridDim.x * blockDim.x * 4) {
+        ushort4 v;
+        if (i < filled) {
+            v = *reinterpret_cast<const ushort4*>(r + reorder_source(i, Pf));
+        } else {
+            const uint16_t v0 = r[0];
+            v = make_ushort4(v0, v0, v0, v0);
+        }
+        *reinterpret_cast<ushort4*>(o + i) = v;
+    }
+}
+
 __global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __restrict__ raw,
                                                           const float* __restrict__ refm,
                                                           const float* __restrict__ dark,
                                                           float* __restrict__ out, int Df, int quads_per_row,
-                                                          int rows, PlaneList planes)
+                                                          int rows, PlaneList planes, int module_major)
 {
     const int f = blockIdx.y;
     const int plane = planes.p[f];
@@ -45,7 +97,18 @@ __global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __rest
         const int y = (int)(q / quads_per_row);
         const int x = (int)(q - (long)y * quads_per_row) * 4;
         const long e = (long)y * Df + x;
-        const ushort4 v = *reinterpret_cast<const ushort4*>(r + e);
+        // module-major: four detectors of one module are four consecutive
+        // samples (e is a multiple of 4 and modules hold 16), or all four
+        // read sample 0 past the table's filled part
+        ushort4 v;
+        if (!module_major) {
+            v = *reinterpret_cast<const ushort4*>(r + e);
+        } else if (e < (long)rows * kRofexDetectors) {
+            v = *reinterpret_cast<const ushort4*>(r + reorder_source(e, rows));
+        } else {
+            const uint16_t v0 = r[0];
+            v = make_ushort4(v0, v0, v0, v0);
+        }
         const float4 d = *reinterpret_cast<const float4*>(rm + e);
         const float4 dkv = *reinterpret_cast<const float4*>(dk + x);
         const float vin[4] = {(float)v.x, (float)v.y, (float)v.z, (float)v.w};
@@ -68,7 +131,7 @@ __global__ void __launch_bounds__(256) attenuation_kernel_scalar(const uint16_t*
                                                                  const float* __restrict__ refm,
                                                                  const float* __restrict__ dark,
                                                                  float* __restrict__ out, int Df, int rows,
-                                                                 PlaneList planes)
+                                                                 PlaneList planes, int module_major)
 {
     const int f = blockIdx.y;
     const int plane = planes.p[f];
@@ -76,7 +139,7 @@ __global__ void __launch_bounds__(256) attenuation_kernel_scalar(const uint16_t*
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int x = (int)(e % Df);
-    float num = (float)raw[(long)f * n + e] - dark[(long)plane * Df + x];
+    float num = (float)raw[(long)f * n + (module_major ? reorder_source(e, rows) : e)] - dark[(long)plane * Df + x];
     if (num < 1.0e-5f) num = 1.0e-5f;
     const float den = refm[(long)plane * n + e];
     out[(long)f * n + e] = -logf(num / fabsf(den)) * (den > 0.0f ? 1.0f : 0.0f);
@@ -89,6 +152,7 @@ struct risa_attenuation_s {
     risa_geom g{};
     risa::DeviceBuffer<float> refm;   // [planes][P_f][D_f]
     risa::DeviceBuffer<float> dark;   // [planes][D_f] (+3 padding for the vector load)
+    int layout = RISA_LAYOUT_DETECTOR_MAJOR;
 };
 
 using namespace risa;
@@ -144,15 +208,16 @@ static void launch_attenuation(risa_attenuation_t h, const uint16_t* raw, const 
         const uint16_t* r = raw + (size_t)f0 * n;
         float* o = out + (size_t)f0 * n;
         const bool vec = (Df % 4 == 0) && ((uintptr_t)r % 8 == 0) && ((uintptr_t)o % 16 == 0);
+        const int mm = h->layout == RISA_LAYOUT_MODULE_MAJOR ? 1 : 0;
         if (vec) {
             const int quads = Df / 4;
             dim3 grid((unsigned)std::min<long>(ceil_div((long)quads * Pf, 256), kAttGrid), nf);
             hipLaunchKernelGGL(attenuation_kernel, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df, quads,
-                               Pf, pl);
+                               Pf, pl, mm);
         } else {
             dim3 grid(ceil_div((long)n, 256), nf);
             hipLaunchKernelGGL(attenuation_kernel_scalar, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df,
-                               Pf, pl);
+                               Pf, pl, mm);
         }
         check_launch("attenuation_kernel");
     }
@@ -175,6 +240,53 @@ extern "C" int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16
         if (!h || !raw || !out || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
         if (frames == 0) return;
         launch_attenuation(h, raw, planes, frames, out, as_stream(stream));
+    });
+}
+
+static void check_rofex_frame(int Pf, int Df)
+{
+    if (Df < kRofexDetectors)
+        fail(RISA_ERR_UNSUPPORTED, "module-major frames need at least %d detectors (27 modules of 16), got %d",
+             kRofexDetectors, Df);
+    if ((long)Pf * Df >= (1L << 31)) fail(RISA_ERR_UNSUPPORTED, "frame too large");
+}
+
+extern "C" int risa_hip_attenuation_set_input_layout(risa_attenuation_t h, int layout)
+{
+    return guarded("risa_hip_attenuation_set_input_layout", [&] {
+        if (!h) fail(RISA_ERR_ARG, "null handle");
+        if (layout != RISA_LAYOUT_DETECTOR_MAJOR && layout != RISA_LAYOUT_MODULE_MAJOR)
+            fail(RISA_ERR_ARG, "unknown input layout %d", layout);
+        if (layout == RISA_LAYOUT_MODULE_MAJOR) check_rofex_frame(h->g.numberOfFanProjections, h->g.numberOfFanDetectors);
+        h->layout = layout;
+    });
+}
+
+extern "C" int risa_hip_attenuation_get_input_layout(risa_attenuation_t h, int* layout)
+{
+    return guarded("risa_hip_attenuation_get_input_layout", [&] {
+        if (!h || !layout) fail(RISA_ERR_ARG, "null argument");
+        *layout = h->layout;
+    });
+}
+
+extern "C" int risa_hip_reorder_run(const uint16_t* raw, int frames, int Pf, int Df, uint16_t* ordered, void* stream)
+{
+    return guarded("risa_hip_reorder_run", [&] {
+        if (!raw || !ordered || frames < 0 || Pf <= 0 || Df <= 0) fail(RISA_ERR_ARG, "bad argument");
+        if (raw == ordered) fail(RISA_ERR_ARG, "the reordering cannot run in place");
+        check_rofex_frame(Pf, Df);
+        if (frames > 65535) fail(RISA_ERR_UNSUPPORTED, "at most 65535 frames per call");
+        if (!frames) return;
+        const long n = (long)Pf * Df;
+        if (n % 4 == 0 && (uintptr_t)raw % 8 == 0 && (uintptr_t)ordered % 8 == 0) {
+            dim3 grid((unsigned)std::min<long>(ceil_div(n / 4, 256), kAttGrid), frames);
+            hipLaunchKernelGGL(reorder4_kernel, grid, dim3(256), 0, as_stream(stream), raw, ordered, n, Pf);
+        } else {
+            dim3 grid((unsigned)std::min<long>(ceil_div(n, 256), kAttGrid), frames);
+            hipLaunchKernelGGL(reorder_kernel, grid, dim3(256), 0, as_stream(stream), raw, ordered, n, Pf);
+        }
+        check_launch("reorder_kernel");
     });
 }
 

@@ -198,6 +198,16 @@ extern "C" int risa_hip_chain_intermediate(risa_chain_t h, int which, const void
     });
 }
 
+extern "C" int risa_hip_chain_set_input_layout(risa_chain_t h, int layout)
+{
+    return guarded("risa_hip_chain_set_input_layout", [&] {
+        if (!h) fail(RISA_ERR_ARG, "null handle");
+        // captured graphs hold the attenuation launches of the layout they saw
+        if (!h->graphs.empty()) fail(RISA_ERR_UNSUPPORTED, "set the chain's input layout before its first graph run");
+        ok(risa_hip_attenuation_set_input_layout(h->att, layout));
+    });
+}
+
 extern "C" int risa_hip_chain_set_variant(risa_chain_t h, int variant)
 {
     return guarded("risa_hip_chain_set_variant", [&] {

@@ -146,6 +146,19 @@ H2D::H2D(const std::string& configFile)
 }
 H2D::~H2D() { stop(); }
 
+// ----------------------------------------------------------- Reordering --
+Reordering::Reordering(const std::string& configFile)
+{
+    const risa_geom g = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_Reordering");
+    const int P = g.numberOfFanProjections, D = g.numberOfFanDetectors;
+    start("Reordering", device_count(), processors_per_device(configFile), pool, fan_elems(g),
+          [P, D](int, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_reorder_run(in.data(), 1, P, D, out.data(), s), "Reordering::processor");
+          });
+}
+Reordering::~Reordering() { stop(); }
+
 // ---------------------------------------------------------- Attenuation --
 Attenuation::Attenuation(const std::string& configFile)
 {

@@ -5,6 +5,7 @@
 // (include/risa_hip.h).
 //
 //   H2D            Copy/H2D.cu:84-169            host uint16 frame -> device
+//   Reordering     Reordering.cu:33-204          ROFEX module-major -> detector-major
 //   Attenuation    Attenuation.cu:48-175         uint16 fan sinogram -> float
 //   Fan2Para       Fan2Para.cu:56-249            fan -> parallel sinogram
 //   Filter         Filter.cu:54-143              in place
@@ -51,6 +52,15 @@ class H2D : public DeviceStage<HostImage<uint16_t>, DeviceImage<uint16_t>> {
 public:
     explicit H2D(const std::string& configFile);
     ~H2D();
+};
+
+// Permutes each ROFEX frame from its module-major layout (27 modules x P_f x
+// 16) to [P_f][D_f] (risa_hip_reorder_run); keys numberOfFanDetectors,
+// memPoolSize_Reordering, samplingRate, scanRate as Reordering::readConfig.
+class Reordering : public DeviceStage<DeviceImage<uint16_t>, DeviceImage<uint16_t>> {
+public:
+    explicit Reordering(const std::string& configFile);
+    ~Reordering();
 };
 
 class Attenuation : public DeviceStage<DeviceImage<uint16_t>, DeviceImage<float>> {

@@ -77,6 +77,24 @@ class Attenuation(_Handle):
         check(load().risa_hip_attenuation_run_batch(self.h, ptr(raw), ptr(pl), len(pl), ptr(out), _stream()))
         return out
 
+    @property
+    def input_layout(self) -> int:
+        v = C.c_int()
+        check(load().risa_hip_attenuation_get_input_layout(self.h, C.byref(v)))
+        return v.value
+
+    @input_layout.setter
+    def input_layout(self, layout: int):
+        """RISA_LAYOUT_DETECTOR_MAJOR (0) or RISA_LAYOUT_MODULE_MAJOR (1)."""
+        check(load().risa_hip_attenuation_set_input_layout(self.h, layout))
+
+
+def reorder(raw, frames: int, Pf: int, Df: int, out):
+    """Module-major ROFEX frames -> detector-major (the Reordering stage)."""
+    _require_cuda(raw, out)
+    check(load().risa_hip_reorder_run(ptr(raw), frames, Pf, Df, ptr(out), _stream()))
+    return out
+
 
 class Fan2Para(_Handle):
     _destroy = "risa_hip_fan2para_destroy"
@@ -284,3 +302,6 @@ class Chain(_Handle):
     @variant.setter
     def variant(self, v: int):
         check(load().risa_hip_chain_set_variant(self.h, v))
+
+    def set_input_layout(self, layout: int):
+        check(load().risa_hip_chain_set_input_layout(self.h, layout))

@@ -1,7 +1,8 @@
 """The C++ glados-style stage pipeline on the GPU (RISA/example/main.cpp's
-wiring): frames streamed host -> device -> host through H2D, Attenuation,
-Fan2Para, Filter, Backprojection (mask fused), optionally Masking, and D2H.
-The slices it saves are checked against the FP64 oracle chain."""
+wiring): frames streamed host -> device -> host through H2D, optionally
+Reordering (module-major ROFEX frames), Attenuation, Fan2Para, Filter,
+Backprojection (mask fused), optionally Masking, and D2H.  The slices it
+saves are checked against the FP64 oracle chain."""
 from __future__ import annotations
 
 import json
@@ -38,7 +39,7 @@ def read_saved(path, N):
     return out
 
 
-@pytest.mark.parametrize("cfg,extra", [("c0", ()), ("c1", ("--masking",))])
+@pytest.mark.parametrize("cfg,extra", [("c0", ()), ("c1", ("--masking",)), ("c0", ("--module-major",))])
 def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extra):
     from risa_amd import synth
 

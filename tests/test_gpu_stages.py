@@ -61,6 +61,90 @@ def test_attenuation_batch_equals_single(risa, geom_c0, frames_c0, cuda):
         assert torch.equal(one, outb[f])
 
 
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_reorder_and_module_major_attenuation(risa, orc, cfg, cuda):
+    """Reordering on the device (Reordering.cu:127-150, 180-204) bit-exact vs
+    the oracle, and the attenuation reading module-major frames through the
+    same permutation bit-exact vs Reordering followed by Attenuation: batched
+    and single-frame launches, 4-wide loads and (misaligned input) the
+    one-sample kernel."""
+    import torch
+    from risa_amd import device, synth
+
+    g = risa.geom_from_config(risa.config_path(cfg))
+    P, D, planes = g.numberOfFanProjections, g.numberOfFanDetectors, g.numberOfPlanes
+    print(f"case: reorder + module-major attenuation {cfg} ({P} x {D})", flush=True)
+    F = 3
+    raw_mm = np.random.default_rng(17).integers(100, 60000, size=(F, P * D), dtype=np.uint16)
+    ref = orc.reorder_u16(raw_mm, P, D)
+    mm = _t(raw_mm, cuda)
+    ordered = _t(np.zeros((F, P, D), np.uint16), cuda)
+    device.reorder(mm, F, P, D, ordered)
+    _sync()
+    np.testing.assert_array_equal(ordered.cpu().numpy(), ref)
+    # a misaligned output takes the one-sample kernel
+    obuf = _t(np.zeros(F * P * D + 1, np.uint16), cuda)
+    device.reorder(mm, F, P, D, obuf[1:])
+    _sync()
+    np.testing.assert_array_equal(obuf[1:].cpu().numpy().reshape(F, P, D), ref)
+
+    avg_ref, avg_dark = synth.prepare_reference(g, synth.reference_frames(g, 4, seed=917))
+    pl = np.array([f % planes for f in range(F)], np.int32)
+    plain = device.Attenuation(g, avg_ref, avg_dark)
+    fused = device.Attenuation(g, avg_ref, avg_dark)
+    fused.input_layout = risa.LAYOUT_MODULE_MAJOR
+    assert fused.input_layout == risa.LAYOUT_MODULE_MAJOR and plain.input_layout == risa.LAYOUT_DETECTOR_MAJOR
+    want = torch.empty((F,) + g.fan_shape, dtype=torch.float32, device=cuda)
+    got = torch.empty_like(want)
+    plain.batch(ordered, pl, want)
+    fused.batch(mm, pl, got)
+    assert torch.equal(got, want)
+    # misaligned module-major input takes the one-sample kernel
+    buf = torch.zeros(F * P * D + 1, dtype=mm.dtype, device=cuda)
+    buf[1:] = mm.reshape(-1)
+    for f in range(F):
+        one = torch.empty(g.fan_shape, dtype=torch.float32, device=cuda)
+        fused(mm[f], int(pl[f]), one)
+        assert torch.equal(one, want[f])
+        fused(buf[1 + f * P * D:1 + (f + 1) * P * D], int(pl[f]), one)
+        assert torch.equal(one, want[f])
+
+
+def test_chain_module_major_input(risa, orc, geom_c0, frames_c0, cuda):
+    """The chain reading ROFEX module-major frames (the reference's H2D ->
+    Reordering -> Attenuation, the reordering fused) gives the images of the
+    detector-major chain bit for bit: plain and mixed, eager and graph; the
+    layout is refused once graphs are captured."""
+    import torch
+    from risa_amd import _lib, device
+
+    g, fs = geom_c0, frames_c0
+    P, D = g.numberOfFanProjections, g.numberOfFanDetectors
+    # module-major frames whose reordering is fs.raw (D = 27 * 16 here)
+    mm = np.stack([f.reshape(P, 27, 16).transpose(1, 0, 2).reshape(-1) for f in fs.raw])
+    np.testing.assert_array_equal(orc.reorder_u16(mm, P, D), fs.raw)
+    raw, raw_mm = _t(fs.raw, cuda), _t(mm, cuda)
+    F = len(fs.planes)
+    for mode in (risa.CHAIN_PLAIN, risa.CHAIN_MIXED):
+        print(f"case: chain module-major input, mode {mode}", flush=True)
+        a = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=mode, max_frames=F)
+        b = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=mode, max_frames=F)
+        b.variant = a.variant
+        b.set_input_layout(risa.LAYOUT_MODULE_MAJOR)
+        s = torch.cuda.Stream()   # graph replays need a non-default stream
+        for use_graph in (False, True):
+            ia = torch.empty((F,) + g.img_shape, dtype=torch.float32, device=cuda)
+            ib = torch.empty_like(ia)
+            _sync()
+            with torch.cuda.stream(s):
+                a(raw, fs.planes, ia, use_graph=use_graph)
+                b(raw_mm, fs.planes, ib, use_graph=use_graph)
+            _sync()
+            assert torch.equal(ia, ib)
+        with pytest.raises(_lib.RisaError, match="RISA_ERR_UNSUPPORTED"):
+            b.set_input_layout(risa.LAYOUT_DETECTOR_MAJOR)
+
+
 def test_fan2para_matches_oracle(risa, orc, geom_c0, frames_c0, cuda):
     import torch
     from risa_amd import device

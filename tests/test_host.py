@@ -131,6 +131,23 @@ def test_status_codes_cross_the_boundary(risa):
     assert lib.risa_last_error()
 
 
+def test_reorder_and_layout_argument_errors(risa):
+    """The reordering's argument checks run before any device work: frames
+    narrower than 27 modules of 16 detectors are refused (the reference's
+    table indexes past them), as are in-place calls and unknown layouts."""
+    from risa_amd import _lib
+
+    lib = _lib.load()
+    dummy = C.c_void_p(4096)
+    assert lib.risa_hip_reorder_run(None, 1, 10, 432, dummy, None) == 1          # RISA_ERR_ARG
+    assert lib.risa_hip_reorder_run(dummy, 1, 10, 432, dummy, None) == 1         # in place
+    assert lib.risa_hip_reorder_run(dummy, 1, 10, 400, C.c_void_p(8192), None) == 4   # RISA_ERR_UNSUPPORTED
+    assert b"432" in lib.risa_last_error()
+    assert lib.risa_hip_reorder_run(dummy, 0, 10, 432, C.c_void_p(8192), None) == 0   # nothing to do
+    assert lib.risa_hip_attenuation_set_input_layout(None, 1) == 1
+    assert lib.risa_hip_chain_set_input_layout(None, 1) == 1
+
+
 def test_backprojector_variant_shapes(risa):
     """The variant table the autotuner walks: every shape is well formed (a
     16-pixel-wide FP16 tile, whole pixel columns per thread, FP16 LDS planes

@@ -190,6 +190,30 @@ def test_attenuation_closed_form_and_clamps(orc, risa):
                - (-math.log((5000 - float(dark[1, 1])) / 1e-5)) * mask[1, 1]) < 1e-6
 
 
+# ------------------------------------------------------- f3 reordering --
+@pytest.mark.parametrize("P,D", [(7, 432), (5, 768), (1, 432)])
+def test_reorder_is_the_module_transpose(orc, P, D):
+    """orc_reorder_u16 (Reordering.cu:180-204).  With 27 modules of 16
+    detectors the table moves [module][projection][16] to
+    [projection][module*16 + detector] (a numpy transpose, independent of the
+    table loops); wider frames keep that order over the first P*432 samples
+    and read sample 0 past it (the zero-initialised table)."""
+    rng = np.random.default_rng(P * 1000 + D)
+    raw = rng.integers(1, 65535, size=(2, P * D), dtype=np.uint16)
+    got = orc.reorder_u16(raw, P, D)
+    for f in range(2):
+        head = raw[f, :P * 432].reshape(27, P, 16).transpose(1, 0, 2).reshape(-1)
+        flat = got[f].reshape(-1)
+        np.testing.assert_array_equal(flat[:P * 432], head)
+        assert (flat[P * 432:] == raw[f, 0]).all()
+    if D == 432:
+        # a permutation: every sample appears once
+        ids = np.arange(P * D, dtype=np.uint16)
+        np.testing.assert_array_equal(np.sort(orc.reorder_u16(ids, P, D).reshape(-1)), ids)
+    with pytest.raises(ValueError):
+        orc.reorder_u16(np.zeros(P * 400, np.uint16), P, 400)
+
+
 # ---------------------------------------------------- a5 rebinning tables --
 def test_ellipse_map_fixed_points_and_identity(orc):
     """ellipse_kreis_uwe (cuda_kernels_fan2para.h:38-62): for any tilt, 0 and
