@@ -119,6 +119,22 @@ def kernel_only(args, g, dev):
                 fn = lambda: att.batch(raw, planes, out)
                 layout = "module-major (reordering fused)" if args.kernel == "att_mm" else "detector-major"
                 what = f"attenuation, {F} frames of {Pf} x {Df}, {layout} input"
+        elif args.kernel == "prep":
+            # flat-field preparation of 8 reference frames per plane (1 % dead
+            # columns): defect detection and repair of each, then the average
+            from risa_amd import synth
+            R = 8
+            dead = synth.dead_columns(g.numberOfFanDetectors, 0.01, 7001)
+            ref_np = synth.reference_frames(g, R, seed=901, dead=dead)
+            t0 = time.perf_counter()
+            synth.prepare_reference(g, ref_np)
+            host_ms = (time.perf_counter() - t0) * 1e3
+            frames = torch.from_numpy(ref_np).to(dev)
+            avg_ref = torch.empty((g.numberOfPlanes,) + g.fan_shape, dtype=torch.float32, device=dev)
+            avg_dark = torch.empty((g.numberOfPlanes, g.numberOfFanDetectors), dtype=torch.float32, device=dev)
+            fn, per = (lambda: device.prepare_reference(g, frames, R, avg_ref, avg_dark)), 1
+            what = (f"flat-field preparation, {R} x {g.numberOfPlanes} frames of {g.numberOfFanProjections} x "
+                    f"{g.numberOfFanDetectors} (host path: {host_ms:.1f} ms on 1 core)")
         else:
             n = 64 << 20
             src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -153,7 +169,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch (96 geometries) and differentiable-FBP legs")
     ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
-    ap.add_argument("--kernel", choices=("bp", "bp16", "att", "att_mm", "reorder", "copy"),
+    ap.add_argument("--kernel", choices=("bp", "bp16", "att", "att_mm", "reorder", "prep", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     args = ap.parse_args()

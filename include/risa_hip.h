@@ -178,6 +178,27 @@ int risa_hip_attenuation_destroy(risa_attenuation_t h);
 int risa_hip_attenuation_set_input_layout(risa_attenuation_t h, int layout);
 int risa_hip_attenuation_get_input_layout(risa_attenuation_t h, int* layout);
 
+/* Flat-field preparation on the device (the host functions above, same
+ * results bit for bit).  Pointers are device memory; the calls run on the
+ * current device, allocate temporary buffers and synchronise `stream`
+ * (set-up paths: never under stream capture).
+ *
+ * risa_hip_repair_defects: the loop of Attenuation::init over reference
+ * frames (Attenuation.cu:190-195): findDefectDetectors +
+ * interpolateDefectDetectors (interpolationFunctions.h:28-97) on `frames`
+ * frames [P][D] in place; the flags [frames][D] go to `flags` when it is not
+ * NULL.  D <= 8190. */
+int risa_hip_repair_defects(uint16_t* frames, int nframes, int P, int D, double threshMin, double threshMax,
+                            int32_t* flags, void* stream);
+/* Attenuation::init's reference part (Attenuation.cu:177-216, computeAverage
+ * :236-251, computeDarkAverage :219-234) on R*planes device frames: repairs
+ * them in place, avg_ref [planes][P_f][D_f], avg_dark [planes][D_f] (0). */
+int risa_hip_attenuation_prepare_reference(const risa_geom* g, uint16_t* ref_frames, int R, float* avg_ref,
+                                           float* avg_dark, void* stream);
+/* risa_hip_attenuation_create with the averages in device memory. */
+int risa_hip_attenuation_create_device(const risa_geom* g, const float* avg_ref, const float* avg_dark, int device,
+                                       void* stream, risa_attenuation_t* out);
+
 /* ------------------------------------------------------------- Reordering -- */
 /* Replaces Reordering::processor's reorder kernel with its createHashTable
  * permutation (Reordering.cu:127-150, 180-204): `frames` consecutive

@@ -155,6 +155,9 @@ _SIGS = {
     "risa_hip_attenuation_set_input_layout": [P, C.c_int],
     "risa_hip_attenuation_get_input_layout": [P, C.POINTER(C.c_int)],
     "risa_hip_reorder_run": [P, C.c_int, C.c_int, C.c_int, P, P],
+    "risa_hip_repair_defects": [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P],
+    "risa_hip_attenuation_prepare_reference": [C.POINTER(Geom), P, C.c_int, P, P, P],
+    "risa_hip_attenuation_create_device": [C.POINTER(Geom), P, P, C.c_int, P, C.POINTER(P)],
     "risa_hip_fan2para_create": [C.POINTER(Geom), C.c_int, C.POINTER(P)],
     "risa_hip_fan2para_create_ex": [C.POINTER(Geom), C.c_int, C.c_int, C.POINTER(P)],
     "risa_hip_fan2para_run": [P, P, C.c_int, P, P],

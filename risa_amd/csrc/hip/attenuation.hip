@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 
 #include "../host/tables.h"
 
@@ -145,6 +146,204 @@ __global__ void __launch_bounds__(256) attenuation_kernel_scalar(const uint16_t*
     out[(long)f * n + e] = -logf(num / fabsf(den)) * (den > 0.0f ? 1.0f : 0.0f);
 }
 
+// ---------------------------------------------------------------------------
+// Flat-field preparation on the device (Attenuation::init, Attenuation.cu:
+// 177-216): defect detection and repair of every reference frame
+// (interpolationFunctions.h:28-97) and the frame average.  The arithmetic is
+// the host restatement's operation for operation: the per-detector scores
+// are sums of integers in double (exact in any order), the products the
+// reference rounds on their own are rounded_mul / rounded_mul_d (no
+// multiply-add contraction), and the repair walks each row's runs in the
+// reference's order.
+
+__device__ __forceinline__ float rounded_mulf(float x, float y) { return __builtin_fmaf(x, y, 0.0f); }
+__device__ __forceinline__ double rounded_mul_d(double x, double y) { return __builtin_fma(x, y, 0.0); }
+
+// Per-detector statistics of findDefectDetectors (interpolationFunctions.h:
+// 36-47) over one chunk of rows: the total variation sum_p |x[p][d] -
+// x[p+1][d]| and max / min of x[p][d] for the chunk's p in [0, P-2].  Grid
+// (detector blocks of 64, frames, kDefectChunks row chunks); a workgroup's 4
+// waves take every 4th row of its chunk and are combined in LDS.  The sums
+// are of integers, exact in double in any order.
+constexpr int kDefectChunks = 8;
+struct DefectPartial {
+    double tv, hi, lo;
+};
+
+__global__ void __launch_bounds__(256) defect_stats_kernel(const uint16_t* __restrict__ frames, int D, int P,
+                                                           DefectPartial* __restrict__ part)
+{
+    __shared__ double s_tv[4][64], s_hi[4][64], s_lo[4][64];
+    const int f = blockIdx.y, z = blockIdx.z, lane = threadIdx.x % 64, grp = threadIdx.x / 64;
+    const int d = blockIdx.x * 64 + lane;
+    const uint16_t* x = frames + (long)f * P * D;
+    const int rows = P - 1, p0 = (int)((long)rows * z / kDefectChunks), p1 = (int)((long)rows * (z + 1) / kDefectChunks);
+    double tv = 0.0, hi = 0.0, lo = 0.0;
+    if (d < D) {
+        hi = lo = (double)x[d];   // row 0 (the only row when P = 1)
+        for (int p = p0 + grp; p < p1; p += 4) {
+            const double a = x[d + (long)p * D], b = x[d + (long)(p + 1) * D];
+            tv += fabs(a - b);
+            hi = fmax(hi, a);
+            lo = fmin(lo, a);
+        }
+    }
+    s_tv[grp][lane] = tv;
+    s_hi[grp][lane] = hi;
+    s_lo[grp][lane] = lo;
+    __syncthreads();
+    if (grp == 0 && d < D) {
+        for (int g = 1; g < 4; g++) {
+            tv += s_tv[g][lane];
+            hi = fmax(hi, s_hi[g][lane]);
+            lo = fmin(lo, s_lo[g][lane]);
+        }
+        part[((long)f * kDefectChunks + z) * D + d] = DefectPartial{tv, hi, lo};
+    }
+}
+
+struct DefectWindow {
+    double w[9];   // the first 9 normalised filterFunction taps (Attenuation.cu:179-182)
+};
+
+// Flags of frame f (findDefectDetectors, interpolationFunctions.h:48-70): the
+// score tv * (max - min)^2 of each detector (from the chunk partials, into
+// LDS) against the window-weighted sum over the detector's half, indices
+// taken modulo D/2 in unsigned arithmetic (Q17); "flickering" ones flag their
+// +-2 neighbours (modulo D, unsigned).  One workgroup per frame; dynamic LDS
+// D doubles.
+__global__ void __launch_bounds__(256) defect_flags_kernel(const DefectPartial* __restrict__ part, int D, double tmin,
+                                                           double tmax, DefectWindow win, int32_t* __restrict__ flags)
+{
+    extern __shared__ double sc[];
+    const int f = blockIdx.x;
+    int32_t* fl = flags + (long)f * D;
+    for (int d = threadIdx.x; d < D; d += blockDim.x) {
+        DefectPartial q = part[(long)f * kDefectChunks * D + d];
+        for (int z = 1; z < kDefectChunks; z++) {
+            const DefectPartial r = part[((long)f * kDefectChunks + z) * D + d];
+            q.tv += r.tv;
+            q.hi = fmax(q.hi, r.hi);
+            q.lo = fmin(q.lo, r.lo);
+        }
+        sc[d] = q.tv * ((q.hi - q.lo) * (q.hi - q.lo));
+        fl[d] = 0;
+    }
+    __syncthreads();
+    const unsigned half = (unsigned)D / 2u;
+    for (unsigned t = threadIdx.x; t < 2u * half; t += blockDim.x) {
+        const unsigned seg = t / half, i = t - seg * half;
+        double ref = 0.0;
+        for (unsigned j = 0; j < 9; j++) {
+            ref = ref + rounded_mul_d(win.w[j], sc[((i - j) % half) + seg * half]);
+            ref = ref + rounded_mul_d(win.w[j], sc[((i + j) % half) + seg * half]);
+        }
+        const unsigned d = seg * half + i;
+        if (sc[d] < tmin * ref) fl[d] = 1;
+        if (sc[d] > tmax * ref)
+            for (int off = -2; off <= 2; off++) fl[(d + (unsigned)off) % (unsigned)D] = 1;
+    }
+}
+
+// Repair of frame f (interpolateDefectDetectors, interpolationFunctions.h:
+// 73-97): every workgroup of the frame lists the runs of flagged detectors
+// in order -- the maximal flagged intervals of [0, D), the last one ending
+// at D (detector 0 rewritten once more) when it reaches D - 1 and detector 0
+// is flagged, as the reference's scan leaves them -- each thread taking a
+// contiguous chunk of detectors at an offset from a prefix sum of the chunks'
+// run counts; then each thread repairs one row, applying the runs in the
+// reference's order so that a run reading a detector an earlier run (or
+// itself) rewrote sees the new value as the sequential loop does.  Grid
+// (frames, row blocks of 256); dynamic LDS: D flags + (D / 2 + 2) runs.
+__global__ void __launch_bounds__(256) defect_repair_kernel(uint16_t* __restrict__ frames, int D, int P,
+                                                            const int32_t* __restrict__ flags)
+{
+    extern __shared__ int smem_i[];
+    int* fl = smem_i;
+    int2* runs = reinterpret_cast<int2*>(smem_i + ((D + 1) & ~1));
+    __shared__ int offs[256];
+    __shared__ int nruns;
+    const int f = blockIdx.x;
+    for (int d = threadIdx.x; d < D; d += blockDim.x) fl[d] = flags[(long)f * D + d] != 0;
+    __syncthreads();
+    const unsigned uD = (unsigned)D;
+    const int per = (D + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int c0 = min(D, (int)threadIdx.x * per), c1 = min(D, c0 + per);
+    int cnt = 0;
+    for (int d = c0; d < c1; d++) cnt += (fl[d] && (d == 0 || !fl[d - 1])) ? 1 : 0;
+    offs[threadIdx.x] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            const int c = offs[t];
+            offs[t] = acc;
+            acc += c;
+        }
+        nruns = acc;
+    }
+    __syncthreads();
+    int n = offs[threadIdx.x];
+    for (int d = c0; d < c1; d++)
+        if (fl[d] && (d == 0 || !fl[d - 1])) {
+            int e = d;
+            while (e + 1 < D && fl[e + 1]) e++;
+            runs[n++] = make_int2(d, (e == D - 1 && fl[0]) ? D : e);
+        }
+    __syncthreads();
+    const int p = blockIdx.y * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint16_t* row = frames + ((long)f * P + p) * D;
+    for (int r = 0; r < nruns; r++) {
+        const unsigned first = (unsigned)runs[r].x, last = (unsigned)runs[r].y;
+        const unsigned left = (first - 1u) % uD, right = (last + 1u) % uD;
+        // the sources are read once unless the run itself rewrites one
+        // (possible only across the wrap)
+        const bool fixed = (left < first || left > last) && (right < first || right > last) &&
+                           (last < uD || (left != 0 && right != 0));
+        float a = (float)row[left], b = (float)row[right];
+        for (unsigned i = first; i <= last; i++) {
+            if (!fixed) {
+                a = (float)row[left];
+                b = (float)row[right];
+            }
+            const float w1 = (float)(((double)((float)i - (float)first) + 1.0) /
+                                     ((double)((float)last - (float)first) + 2.0));
+            const float w0 = (float)(1.0 - (double)w1);
+            row[i % uD] = (uint16_t)(rounded_mulf(w0, a) + rounded_mulf(w1, b));
+        }
+    }
+}
+
+// computeAverage (Attenuation.cu:236-251): plane k of reference frame i is
+// block (i + k) (Q2); frames are added in ascending order, each term rounded.
+__global__ void __launch_bounds__(256) reference_average_kernel(const uint16_t* __restrict__ frames, long n, int R,
+                                                                int planes, float factor, float* __restrict__ avg)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n * planes; e += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(e / n);
+        const long q = e - (long)k * n;
+        float acc = 0.0f;
+        for (int i = 0; i < R; i++) acc = acc + rounded_mulf((float)frames[(long)(i + k) * n + q], factor);
+        avg[e] = acc;
+    }
+}
+
+// refm [planes][P_f][D_f] from device averages: +-max(ref - dark, 1e-5) by the
+// area mask (the table risa_hip_attenuation_create builds on the host).
+__global__ void __launch_bounds__(256) refm_kernel(const float* __restrict__ avg_ref, const float* __restrict__ avg_dark,
+                                                   const float* __restrict__ mask, long n, int D, int planes,
+                                                   float* __restrict__ refm)
+{
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n * planes; e += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(e / n);
+        const long q = e - (long)k * n;
+        float den = avg_ref[e] - avg_dark[(long)k * D + q % D];
+        if (den < 1.0e-5f) den = 1.0e-5f;
+        refm[e] = mask[q] != 0.0f ? den : -den;
+    }
+}
+
 }  // namespace risa
 
 struct risa_attenuation_s {
@@ -240,6 +439,102 @@ extern "C" int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16
         if (!h || !raw || !out || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
         if (frames == 0) return;
         launch_attenuation(h, raw, planes, frames, out, as_stream(stream));
+    });
+}
+
+// findDefectDetectors + interpolateDefectDetectors on `frames` device frames
+// [P][D], flags into `flags` (device, frames * D).
+static void repair_frames(uint16_t* frames, int nframes, int P, int D, double tmin, double tmax, int32_t* flags,
+                          hipStream_t s)
+{
+    static const double window[17] = {0.5, 1.0, 1.0, 1.0, 1.5, 2.0, 3.0, 3.5, 2.0,
+                                      3.5, 3.0, 2.0, 1.5, 1.0, 1.0, 1.0, 0.5};
+    double wsum = 0.0;
+    for (double w : window) wsum += w;
+    DefectWindow win{};
+    for (int j = 0; j < 9; j++) win.w[j] = (1.0 / wsum) * window[j];
+    const size_t lds = (size_t)((D + 1) & ~1) * sizeof(int) + (size_t)(D / 2 + 2) * sizeof(int2);
+    if (lds > 64 * 1024 || (size_t)D * sizeof(double) > 64 * 1024)
+        fail(RISA_ERR_UNSUPPORTED, "defect repair: %d detectors exceed the LDS tables", D);
+    DeviceBuffer<DefectPartial> part((size_t)nframes * kDefectChunks * D);   // freed after the synchronisation below
+    hipLaunchKernelGGL(defect_stats_kernel, dim3(ceil_div(D, 64), nframes, kDefectChunks), dim3(256), 0, s, frames, D,
+                       P, part.ptr);
+    check_launch("defect_stats_kernel");
+    hipLaunchKernelGGL(defect_flags_kernel, dim3(nframes), dim3(256), (size_t)D * sizeof(double), s, part.ptr, D,
+                       tmin, tmax, win, flags);
+    check_launch("defect_flags_kernel");
+    hipLaunchKernelGGL(defect_repair_kernel, dim3(nframes, ceil_div(P, 256)), dim3(256), lds, s, frames, D, P, flags);
+    check_launch("defect_repair_kernel");
+    RISA_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+extern "C" int risa_hip_repair_defects(uint16_t* frames, int nframes, int P, int D, double threshMin, double threshMax,
+                                       int32_t* flags, void* stream)
+{
+    return guarded("risa_hip_repair_defects", [&] {
+        if (!frames || nframes < 0 || P <= 0 || D < 2) fail(RISA_ERR_ARG, "bad argument");
+        if (nframes > 65535) fail(RISA_ERR_UNSUPPORTED, "at most 65535 frames per call");
+        if (!nframes) return;
+        hipStream_t s = as_stream(stream);
+        if (flags) {
+            repair_frames(frames, nframes, P, D, threshMin, threshMax, flags, s);
+        } else {
+            DeviceBuffer<int32_t> fl((size_t)nframes * D);
+            repair_frames(frames, nframes, P, D, threshMin, threshMax, fl.ptr, s);
+        }
+    });
+}
+
+extern "C" int risa_hip_attenuation_prepare_reference(const risa_geom* g, uint16_t* ref_frames, int R, float* avg_ref,
+                                                      float* avg_dark, void* stream)
+{
+    return guarded("risa_hip_attenuation_prepare_reference", [&] {
+        if (!g || !ref_frames || !avg_ref || !avg_dark || R < 1) fail(RISA_ERR_ARG, "bad argument");
+        check_geom(g, true, false);
+        const int D = g->numberOfFanDetectors, P = g->numberOfFanProjections, planes = g->numberOfPlanes;
+        if (R * planes > 65535) fail(RISA_ERR_UNSUPPORTED, "too many reference frames");
+        hipStream_t s = as_stream(stream);
+        {
+            DeviceBuffer<int32_t> fl((size_t)R * planes * D);
+            repair_frames(ref_frames, R * planes, P, D, g->threshMin, g->threshMax, fl.ptr, s);
+        }
+        const long n = (long)P * D;
+        const float factor = (float)(1.0 / (float)R);
+        hipLaunchKernelGGL(reference_average_kernel, dim3((unsigned)std::min<long>(ceil_div(n * planes, 256), 4096)),
+                           dim3(256), 0, s, ref_frames, n, R, planes, factor, avg_ref);
+        check_launch("reference_average_kernel");
+        // computeDarkAverage (:219-234) scales by a factor overwritten with 0
+        RISA_HIP_CHECK(hipMemsetAsync(avg_dark, 0, sizeof(float) * (size_t)planes * D, s));
+    });
+}
+
+extern "C" int risa_hip_attenuation_create_device(const risa_geom* g, const float* avg_ref, const float* avg_dark,
+                                                  int device, void* stream, risa_attenuation_t* out)
+{
+    return guarded("risa_hip_attenuation_create_device", [&] {
+        if (!g || !avg_ref || !avg_dark || !out) fail(RISA_ERR_ARG, "null argument");
+        check_geom(g, true, false);
+        DeviceGuard dg(device);
+        hipStream_t s = as_stream(stream);
+        const int Df = g->numberOfFanDetectors, Pf = g->numberOfFanProjections, planes = g->numberOfPlanes;
+        const long n = (long)Df * Pf;
+        std::unique_ptr<risa_attenuation_s> h(new risa_attenuation_s());
+        h->device = device;
+        h->g = *g;
+        const std::vector<float> mask = area_mask(*g);
+        DeviceBuffer<float> dmask;
+        dmask.resize(mask.size());
+        dmask.upload(mask);
+        h->refm.resize((size_t)n * planes);
+        h->dark.resize((size_t)planes * Df + 4);
+        RISA_HIP_CHECK(hipMemsetAsync(h->dark.ptr, 0, sizeof(float) * h->dark.count, s));
+        RISA_HIP_CHECK(hipMemcpyAsync(h->dark.ptr, avg_dark, sizeof(float) * (size_t)planes * Df,
+                                      hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(refm_kernel, dim3((unsigned)std::min<long>(ceil_div(n * planes, 256), 4096)), dim3(256), 0,
+                           s, avg_ref, avg_dark, dmask.ptr, n, Df, planes, h->refm.ptr);
+        check_launch("refm_kernel");
+        RISA_HIP_CHECK(hipStreamSynchronize(s));
+        *out = h.release();
     });
 }
 

@@ -59,9 +59,15 @@ def _require_cuda(*tensors):
 class Attenuation(_Handle):
     _destroy = "risa_hip_attenuation_destroy"
 
-    def __init__(self, g: Geom, avg_ref: np.ndarray, avg_dark: np.ndarray, device=0):
+    def __init__(self, g: Geom, avg_ref, avg_dark, device=0):
+        """avg_ref / avg_dark: host arrays, or device tensors (the table is then built on the device)."""
         super().__init__()
         self.g = g
+        if hasattr(avg_ref, "is_cuda") and avg_ref.is_cuda:
+            _require_cuda(avg_dark)
+            check(load().risa_hip_attenuation_create_device(C.byref(g), ptr(avg_ref), ptr(avg_dark), _dev(device),
+                                                            _stream(), C.byref(self.h)))
+            return
         avg_ref = np.ascontiguousarray(avg_ref, dtype=np.float32)
         avg_dark = np.ascontiguousarray(avg_dark, dtype=np.float32)
         check(load().risa_hip_attenuation_create(C.byref(g), ptr(avg_ref), ptr(avg_dark), _dev(device), C.byref(self.h)))
@@ -87,6 +93,22 @@ class Attenuation(_Handle):
     def input_layout(self, layout: int):
         """RISA_LAYOUT_DETECTOR_MAJOR (0) or RISA_LAYOUT_MODULE_MAJOR (1)."""
         check(load().risa_hip_attenuation_set_input_layout(self.h, layout))
+
+
+def repair_defects(frames, nframes: int, P: int, D: int, thresh_min: float, thresh_max: float, flags=None):
+    """Defect detection and repair of device frames [nframes][P][D] in place."""
+    _require_cuda(frames, flags)
+    check(load().risa_hip_repair_defects(ptr(frames), nframes, P, D, thresh_min, thresh_max,
+                                         ptr(flags) if flags is not None else None, _stream()))
+    return frames
+
+
+def prepare_reference(g: Geom, ref_frames, R: int, avg_ref, avg_dark):
+    """Flat-field preparation of R*planes device reference frames (repaired in place)."""
+    _require_cuda(ref_frames, avg_ref, avg_dark)
+    check(load().risa_hip_attenuation_prepare_reference(C.byref(g), ptr(ref_frames), R, ptr(avg_ref), ptr(avg_dark),
+                                                        _stream()))
+    return avg_ref, avg_dark
 
 
 def reorder(raw, frames: int, Pf: int, Df: int, out):

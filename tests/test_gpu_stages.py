@@ -8,7 +8,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import disk_mask, rel_rmse
+from conftest import ROOT, disk_mask, rel_rmse
 
 pytestmark = pytest.mark.gpu
 
@@ -143,6 +143,71 @@ def test_chain_module_major_input(risa, orc, geom_c0, frames_c0, cuda):
             assert torch.equal(ia, ib)
         with pytest.raises(_lib.RisaError, match="RISA_ERR_UNSUPPORTED"):
             b.set_input_layout(risa.LAYOUT_DETECTOR_MAJOR)
+
+
+def test_defect_repair_matches_reference_vectors(risa, cuda):
+    """Defect detection and repair on the device against the vectors written
+    by the reference's own interpolationFunctions.h:28-97 (tests/golden/),
+    bit-exact, three frames per call; the cases include the unsigned-wrap
+    neighbours at detectors 0 and D-1 and a run wrapping across the seam."""
+    from risa_amd import device
+
+    z = np.load(ROOT / "tests" / "golden" / "defects.npz", allow_pickle=False)
+    for name in z["names"]:
+        data, thr = z[f"{name}__data"], z[f"{name}__thresh"]
+        P, D = data.shape
+        print(f"case: defect repair {name} ({P} x {D})", flush=True)
+        frames = _t(np.stack([data] * 3), cuda)
+        flags = _t(np.full((3, D), 7, np.int32), cuda)
+        device.repair_defects(frames, 3, P, D, float(thr[0]), float(thr[1]), flags)
+        _sync()
+        for f in range(3):
+            np.testing.assert_array_equal(flags[f].cpu().numpy(), z[f"{name}__flags"])
+            np.testing.assert_array_equal(frames[f].cpu().numpy(), z[f"{name}__repaired"])
+
+
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_prepare_reference_on_device_matches_host(risa, cfg, cuda):
+    """Flat-field preparation on the device (repair of every reference frame,
+    the Q2 block average, the zero dark average) equals the host path bit for
+    bit on seeded flat fields with dead columns, and an attenuation handle
+    built from the device averages gives the host-built handle's output."""
+    import torch
+    from risa_amd import device, synth
+
+    g = risa.geom_from_config(risa.config_path(cfg))
+    P, D, planes, R = g.numberOfFanProjections, g.numberOfFanDetectors, g.numberOfPlanes, 4
+    print(f"case: device flat-field preparation {cfg}", flush=True)
+    dead = synth.dead_columns(D, 0.02, 31)
+    ref = synth.reference_frames(g, R, seed=77, dead=dead)
+    avg_ref_h, avg_dark_h = synth.prepare_reference(g, ref)
+    # the host path's repaired frames (its outputs held in named arrays: the
+    # call writes through their addresses)
+    import ctypes
+    from risa_amd import _lib
+    rep_h = ref.copy()
+    out_ref, out_dark = np.empty_like(avg_ref_h), np.empty_like(avg_dark_h)
+    _lib.check(_lib.load().risa_attenuation_prepare_reference(ctypes.byref(g), rep_h.ctypes.data, R,
+                                                               out_ref.ctypes.data, out_dark.ctypes.data))
+    np.testing.assert_array_equal(out_ref, avg_ref_h)
+    frames = _t(ref, cuda)
+    avg_ref = torch.empty(avg_ref_h.shape, dtype=torch.float32, device=cuda)
+    avg_dark = torch.full(avg_dark_h.shape, 5.0, dtype=torch.float32, device=cuda)
+    device.prepare_reference(g, frames, R, avg_ref, avg_dark)
+    _sync()
+    assert (rep_h != ref).any()   # the dead columns were repaired
+    np.testing.assert_array_equal(frames.cpu().numpy(), rep_h)
+    np.testing.assert_array_equal(avg_ref.cpu().numpy(), avg_ref_h)
+    np.testing.assert_array_equal(avg_dark.cpu().numpy(), avg_dark_h)
+    raw = _t(synth.fan_frame(g, planes - 1, 3, 5, dead=dead), cuda)
+    a = device.Attenuation(g, avg_ref_h, avg_dark_h)
+    b = device.Attenuation(g, avg_ref, avg_dark)
+    oa = torch.empty(g.fan_shape, dtype=torch.float32, device=cuda)
+    ob = torch.empty_like(oa)
+    a(raw, planes - 1, oa)
+    b(raw, planes - 1, ob)
+    _sync()
+    assert torch.equal(oa, ob)
 
 
 def test_fan2para_matches_oracle(risa, orc, geom_c0, frames_c0, cuda):
