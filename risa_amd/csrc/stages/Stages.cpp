@@ -160,15 +160,51 @@ Reordering::Reordering(const std::string& configFile)
 Reordering::~Reordering() { stop(); }
 
 // ---------------------------------------------------------- Attenuation --
+// Device memory released at scope exit (set-up only).
+template <class T>
+struct SetupBuffer {
+    T* p = nullptr;
+    explicit SetupBuffer(std::size_t n) { hip_check(hipMalloc((void**)&p, n * sizeof(T)), "hipMalloc"); }
+    ~SetupBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    SetupBuffer(const SetupBuffer&) = delete;
+    SetupBuffer& operator=(const SetupBuffer&) = delete;
+};
+
+// Attenuation::init (Attenuation.cu:177-216) with the reference frames' defect
+// repair and average done on each device (risa_hip_attenuation_prepare_reference).
 Attenuation::Attenuation(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
     const int R = config_int(configFile, "numberOfReferenceFrames");
-    std::vector<uint16_t> frames = reference_frames(configFile, g_, R);
-    std::vector<float> ref((std::size_t)g_.numberOfPlanes * fan_elems(g_));
-    std::vector<float> dark((std::size_t)g_.numberOfPlanes * g_.numberOfFanDetectors);
-    check(risa_attenuation_prepare_reference(&g_, frames.data(), R, ref.data(), dark.data()), "Attenuation::init");
-    init(configFile, ref, dark);
+    const std::vector<uint16_t> frames = reference_frames(configFile, g_, R);
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    const int pool = config_int(configFile, "memPoolSize_attenuation");
+    const std::size_t nref = (std::size_t)g_.numberOfPlanes * fan_elems(g_);
+    const std::size_t ndark = (std::size_t)g_.numberOfPlanes * g_.numberOfFanDetectors;
+    try {
+        for (int d = 0; d < devices; d++) {
+            glados::hip::ScopedDevice sd(d);
+            SetupBuffer<uint16_t> dframes(frames.size());
+            SetupBuffer<float> dref(nref), ddark(ndark);
+            hip_check(hipMemcpy(dframes.p, frames.data(), frames.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+                      "hipMemcpy");
+            check(risa_hip_attenuation_prepare_reference(&g_, dframes.p, R, dref.p, ddark.p, nullptr),
+                  "Attenuation::init");
+            for (int l = 0; l < lanes; l++) {
+                risa_attenuation_t h = nullptr;
+                check(risa_hip_attenuation_create_device(&g_, dref.p, ddark.p, d, nullptr, &h), "Attenuation");
+                h_.push_back(h);
+            }
+        }
+    } catch (...) {
+        for (auto h : h_) risa_hip_attenuation_destroy(h);
+        h_.clear();
+        throw;
+    }
+    start_processors(configFile, pool, devices, lanes);
 }
 
 Attenuation::Attenuation(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark)
@@ -189,6 +225,11 @@ void Attenuation::init(const std::string& configFile, const std::vector<float>& 
         check(risa_hip_attenuation_create(&g_, ref.data(), dark.data(), w / lanes, &h), "Attenuation");
         h_.push_back(h);
     }
+    start_processors(configFile, pool, devices, lanes);
+}
+
+void Attenuation::start_processors(const std::string&, int pool, int devices, int lanes)
+{
     start("Attenuation", devices, lanes, pool, fan_elems(g_),
           [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
               check(risa_hip_attenuation_run(h_[(std::size_t)w], in.data(), (int)in.plane(), out.data(), s),

@@ -73,6 +73,7 @@ public:
 
 private:
     void init(const std::string& configFile, const std::vector<float>& ref, const std::vector<float>& dark);
+    void start_processors(const std::string& configFile, int pool, int devices, int lanes);
     risa_geom g_{};
     std::vector<risa_attenuation_t> h_;
 };
