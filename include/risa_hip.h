@@ -199,6 +199,17 @@ int risa_hip_attenuation_prepare_reference(const risa_geom* g, uint16_t* ref_fra
 int risa_hip_attenuation_create_device(const risa_geom* g, const float* avg_ref, const float* avg_dark, int device,
                                        void* stream, risa_attenuation_t* out);
 
+/* ----------------------------------------------------------------- Output -- */
+/* TIFF layouts of the output side (OfflineSaver.cu:135-178): the offline
+ * image sequence is opened as BigTIFF (TIFFOpen(path, "w8")), the online
+ * latest image as big-endian classic TIFF ("wb"). */
+enum risa_tiff_format { RISA_TIFF_BIGTIFF = 0, RISA_TIFF_CLASSIC_BE = 1, RISA_TIFF_CLASSIC_LE = 2 };
+/* Writes `count` float32 images [height][width] (host memory) as the pages
+ * of one uncompressed TIFF file with the tags OfflineSaver::writeToTiff sets
+ * (OfflineSaver.cu:160-178); replaces its libtiff calls in writeTiffSequence
+ * (:135-158) and the online branch of saveImage (:118-131). */
+int risa_tiff_write_f32(const char* path, const float* images, int count, int width, int height, int format);
+
 /* ------------------------------------------------------------- Reordering -- */
 /* Replaces Reordering::processor's reorder kernel with its createHashTable
  * permutation (Reordering.cu:127-150, 180-204): `frames` consecutive

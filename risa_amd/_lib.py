@@ -155,6 +155,7 @@ _SIGS = {
     "risa_hip_attenuation_set_input_layout": [P, C.c_int],
     "risa_hip_attenuation_get_input_layout": [P, C.POINTER(C.c_int)],
     "risa_hip_reorder_run": [P, C.c_int, C.c_int, C.c_int, P, P],
+    "risa_tiff_write_f32": [C.c_char_p, P, C.c_int, C.c_int, C.c_int, C.c_int],
     "risa_hip_repair_defects": [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P],
     "risa_hip_attenuation_prepare_reference": [C.POINTER(Geom), P, C.c_int, P, P, P],
     "risa_hip_attenuation_create_device": [C.POINTER(Geom), P, P, C.c_int, P, C.POINTER(P)],

@@ -8,11 +8,13 @@
 // recorded frames from RAM); the sink counts slices, measures latency and
 // keeps the first few slices for checking.
 //
-//   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major]
+//   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] [--tiff]
 //
 // --module-major: the source holds ROFEX module-major frames (27 modules x
 // P_f x 16, D_f = 432) and a Reordering stage follows H2D, as
 // RISA/example/main.cpp:112-126 wires it.
+// --tiff: the sink also hands every slice to risa::OfflineSaver, which writes
+// TIFF files as the config's output keys say (outputPath, mode, ...).
 //
 // Prints one JSON line: frames, seconds, slices/s, mean and max latency.
 // --save writes K slices as records {int32 index, int32 frame, int32 plane,
@@ -24,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -122,21 +125,29 @@ class StatsSaver {
 public:
     using manager_type = glados::hip::HostMemoryManager<float>;
 
-    explicit StatsSaver(SinkState* st) : st_(st) {}
+    StatsSaver(SinkState* st, const std::string& tiffConfig) : st_(st)
+    {
+        if (!tiffConfig.empty()) tiff_.reset(new risa::OfflineSaver(tiffConfig));
+    }
 
-    void saveImage(glados::Image<manager_type>&& img, const std::string&)
+    void saveImage(glados::Image<manager_type>&& img, const std::string& path)
     {
         const double lat = img.duration();
-        std::lock_guard<std::mutex> lock(st_->m);
-        st_->count++;
-        st_->lat_sum += lat;
-        st_->lat_max = std::max(st_->lat_max, lat);
-        st_->last = std::chrono::high_resolution_clock::now();
-        if ((long)img.index() < st_->keep) st_->kept[img.index()] = std::vector<float>(img.data(), img.data() + img.size());
+        {
+            std::lock_guard<std::mutex> lock(st_->m);
+            st_->count++;
+            st_->lat_sum += lat;
+            st_->lat_max = std::max(st_->lat_max, lat);
+            st_->last = std::chrono::high_resolution_clock::now();
+            if ((long)img.index() < st_->keep)
+                st_->kept[img.index()] = std::vector<float>(img.data(), img.data() + img.size());
+        }
+        if (tiff_) tiff_->saveImage(std::move(img), path);
     }
 
 private:
     SinkState* st_;
+    std::unique_ptr<risa::OfflineSaver> tiff_;
 };
 
 }  // namespace
@@ -145,14 +156,15 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         std::fprintf(stderr,
-                     "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major]\n",
+                     "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] "
+                     "[--tiff]\n",
                      argv[0]);
         return 2;
     }
     const std::string config = argv[1];
     long frames = 200;
     int unique = 8, keep = 0;
-    bool masking = false, module_major = false;
+    bool masking = false, module_major = false, tiff = false;
     std::string save_path;
     for (int i = 2; i < argc; i++) {
         const std::string a = argv[i];
@@ -163,6 +175,7 @@ int main(int argc, char** argv)
             save_path = argv[++i];
         } else if (a == "--masking") masking = true;
         else if (a == "--module-major") module_major = true;
+        else if (a == "--tiff") tiff = true;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -187,7 +200,7 @@ int main(int argc, char** argv)
         auto backprojection = pipeline.create<Stage<risa::Backprojection>>(config);
         auto masking_stage = masking ? pipeline.create<Stage<risa::Masking>>(config) : nullptr;
         auto d2h = pipeline.create<Stage<risa::D2H>>(config);
-        auto sink = pipeline.create<Sink>("", "IMG", &st);
+        auto sink = pipeline.create<Sink>("", "IMG", &st, tiff ? config : std::string());
 
         pipeline.connect(source, h2d);
         if (reordering) {

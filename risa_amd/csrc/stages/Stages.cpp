@@ -353,4 +353,74 @@ D2H::D2H(const std::string& configFile)
 }
 D2H::~D2H() { stop(); }
 
+// ---------------------------------------------------------- OfflineSaver --
+static std::string config_string(const std::string& path, const char* key)
+{
+    char buf[4096];
+    check(risa_config_lookup_string(path.c_str(), key, buf, sizeof(buf)), key);
+    return std::string(buf);
+}
+
+OfflineSaver::OfflineSaver(const std::string& configFile)
+{
+    N_ = config_int(configFile, "numberOfPixels");
+    outputPath_ = config_string(configFile, "outputPath");
+    fileName_ = config_string(configFile, "outputFileName");
+    (void)config_int(configFile, "numberOfDataFrames");   // read, as the reference does, and unused
+    planes_ = config_int(configFile, "numberOfPlanes");
+    bufferSize_ = config_int(configFile, "outputBufferSize");
+    const std::string mode = config_string(configFile, "mode");
+    if (mode == "offline") online_ = false;
+    else if (mode == "online") online_ = true;
+    else throw std::runtime_error("OfflineSaver: requested mode \"" + mode + "\" not supported");
+    if (N_ < 1 || planes_ < 1 || bufferSize_ < 1) throw std::runtime_error("OfflineSaver: bad output configuration");
+    buffers_.resize((std::size_t)planes_);
+    fileIndex_.assign((std::size_t)planes_, 0u);
+    last_.assign((std::size_t)planes_, std::chrono::steady_clock::now());
+}
+
+OfflineSaver::~OfflineSaver()
+{
+    for (int k = 0; k < planes_; k++) {
+        try {
+            writeSequence(k);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "OfflineSaver: %s\n", e.what());
+        }
+    }
+}
+
+void OfflineSaver::saveImage(glados::Image<manager_type> image, std::string)
+{
+    const int k = (int)image.plane();
+    if (k < 0 || k >= planes_) throw std::runtime_error("OfflineSaver: image of an unknown plane");
+    if (image.size() < (std::size_t)N_ * N_) throw std::runtime_error("OfflineSaver: image smaller than N x N");
+    const double latency = image.duration();
+    if (latency < minLatency_) minLatency_ = latency;
+    if (latency > maxLatency_) maxLatency_ = latency;
+    if (!online_) {
+        buffers_[(std::size_t)k].emplace_back(image.data(), image.data() + (std::size_t)N_ * N_);
+        if ((int)buffers_[(std::size_t)k].size() >= bufferSize_) writeSequence(k);
+        return;
+    }
+    const auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - last_[(std::size_t)k]).count() < 0.02) return;
+    const std::string path = outputPath_ + fileName_ + "_plane" + std::to_string(k) + "_0.tif";
+    check(risa_tiff_write_f32(path.c_str(), image.data(), 1, N_, N_, RISA_TIFF_CLASSIC_BE), "OfflineSaver");
+    last_[(std::size_t)k] = std::chrono::steady_clock::now();
+}
+
+void OfflineSaver::writeSequence(int k)
+{
+    auto& buf = buffers_[(std::size_t)k];
+    if (buf.empty()) return;
+    const std::string path = outputPath_ + fileName_ + "_plane" + std::to_string(k) + "_" +
+                             std::to_string(fileIndex_[(std::size_t)k]) + "seq.tif";
+    std::vector<float> pages(buf.size() * (std::size_t)N_ * N_);
+    for (std::size_t i = 0; i < buf.size(); i++) std::copy(buf[i].begin(), buf[i].end(), pages.begin() + i * N_ * N_);
+    check(risa_tiff_write_f32(path.c_str(), pages.data(), (int)buf.size(), N_, N_, RISA_TIFF_BIGTIFF), "OfflineSaver");
+    ++fileIndex_[(std::size_t)k];
+    buf.clear();
+}
+
 }  // namespace risa

@@ -12,8 +12,10 @@
 //   Backprojection Backprojection.cu:46-208      parallel sinogram -> slice
 //   Masking        Masking.cu:46-169             in place
 //   D2H            Copy/D2H.cu:83-148            device slice -> host
+//   OfflineSaver   OfflineSaver.cu:66-178        sink: TIFF sequences / latest image
 #pragma once
 
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -122,6 +124,34 @@ class D2H : public DeviceStage<DeviceImage<float>, HostImage<float>> {
 public:
     explicit D2H(const std::string& configFile);
     ~D2H();
+};
+
+// Saver for glados::pipeline::SinkStage (OfflineSaver.cu:66-178), keys
+// numberOfPixels, outputPath, outputFileName, numberOfDataFrames,
+// numberOfPlanes, outputBufferSize and mode as OfflineSaver::readConfig.
+// "offline": images are kept per plane and written as a multi-page BigTIFF
+// <outputPath><outputFileName>_plane<k>_<n>seq.tif whenever outputBufferSize
+// have gathered, the rest at destruction.  "online": at most every 20 ms per
+// plane the newest image replaces <outputPath><outputFileName>_plane<k>_0.tif
+// (big-endian classic TIFF).
+class OfflineSaver {
+public:
+    using manager_type = glados::hip::HostMemoryManager<float>;
+    explicit OfflineSaver(const std::string& configFile);
+    ~OfflineSaver();
+    void saveImage(glados::Image<manager_type> image, std::string path);
+    double minLatency() const { return minLatency_; }
+    double maxLatency() const { return maxLatency_; }
+
+private:
+    void writeSequence(int plane);
+    int N_ = 0, planes_ = 0, bufferSize_ = 0;
+    bool online_ = false;
+    std::string outputPath_, fileName_;
+    std::vector<std::vector<std::vector<float>>> buffers_;
+    std::vector<unsigned> fileIndex_;
+    std::vector<std::chrono::steady_clock::time_point> last_;
+    double minLatency_ = 1e30, maxLatency_ = 0.0;
 };
 
 }  // namespace risa

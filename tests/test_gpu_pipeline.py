@@ -64,3 +64,45 @@ def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extr
         assert err < 1e-5
         assert (img[~m] == g.maskingValue).all()
     oc.close()
+
+
+@pytest.mark.parametrize("mode", ["offline", "online"])
+def test_stage_pipeline_tiff_output(risa, orc, cuda, tmp_path, mode):
+    """The pipeline's sink writing TIFF through risa::OfflineSaver
+    (OfflineSaver.cu:66-178): offline, BigTIFF sequences of outputBufferSize
+    pages per plane (the remainder at the end); online, the latest slice per
+    plane.  Every page read back (Pillow) is one of the replayed frames'
+    reconstructions (FP64 oracle, rel RMSE < 1e-5)."""
+    from PIL import Image, ImageSequence
+    from risa_amd import synth
+
+    src = risa.config_path("c0").read_text()
+    src = src.replace('outputPath = "Reco/"', f'outputPath = "{tmp_path}/"')
+    src = src.replace('mode = "offline"', f'mode = "{mode}"').replace("outputBufferSize = 100", "outputBufferSize = 5")
+    cfg = tmp_path / "out.cfg"
+    cfg.write_text(src)
+    g = risa.geom_from_config(cfg)
+    # online mode writes at most every 20 ms: enough frames to outlast that
+    N, unique, frames = g.numberOfPixels, 4, (12 if mode == "offline" else 2000)
+    res = run_pipeline(cfg, frames, 0, tmp_path / "unused.bin", ("--tiff",))
+    assert res["frames"] == frames
+    avg_ref, avg_dark = synth.prepare_reference(g, synth.reference_frames(g, 8, seed=901))
+    oc = orc.Chain(g, avg_ref, avg_dark)
+    m = disk_mask(N)
+    refs = [oc.f64(synth.fan_frame(g, 0, 1000 + u, 55 + u), 0) for u in range(unique)]
+    oc.close()
+    if mode == "offline":
+        names = [f"reco_plane0_{i}seq.tif" for i in range(3)]
+        counts = [5, 5, 2]
+    else:
+        names, counts = ["reco_plane0_0.tif"], [1]
+    assert sorted(p.name for p in tmp_path.glob("reco_*.tif")) == sorted(names)
+    for name, cnt in zip(names, counts):
+        pages = [np.array(pg, dtype=np.float32) for pg in ImageSequence.Iterator(Image.open(tmp_path / name))]
+        assert len(pages) == cnt
+        for img in pages:
+            assert img.shape == (N, N)
+            err = min(rel_rmse(img, r, m) for r in refs)
+            print(f"  {name}: rel RMSE {err:.2e}", flush=True)
+            assert err < 1e-5
+
