@@ -726,10 +726,10 @@ __global__ void __launch_bounds__(256) transpose_kernel(const float* __restrict_
 
 // Ray-driven projector: the exact transpose of bp_linear_kernel.
 //
-// A workgroup serves one angle and kAdjK consecutive detectors, one per lane;
-// its four waves split the image lines (wave w takes lines w, w + 4, ...) and
-// their partial sums are added in wave order, so the result does not depend
-// on scheduling.  Each pixel's t is formed exactly as the forward kernel
+// A workgroup serves one angle and kAdjK consecutive detectors, kAdjDPL per lane;
+// its kAdjWaves waves split the image lines (wave w takes lines w,
+// w + kAdjWaves, ...) and their partial sums are added in wave order, so the
+// result does not depend on scheduling.  Each pixel's t is formed exactly as the forward kernel
 // forms it for the tile (TX x TY) the pixel falls in -- relative to that
 // tile's integer origin T, tabulated per angle in LDS -- so the weights are
 // the forward's own and <BP s, x> = <s, BP^T x> holds to accumulation
@@ -741,7 +741,9 @@ __global__ void __launch_bounds__(256) transpose_kernel(const float* __restrict_
 // the tile-relative t).  Lines run along x when |cos| >= |sin|, else along y,
 // read as rows of the transposed image imgT so a wave's loads stay close
 // together.  ltx, lty: log2 tile edges.
-constexpr int kAdjK = 64;
+constexpr int kAdjDPL = 2;             // detectors per lane: k0 + lane + 64 j
+constexpr int kAdjWaves = 8;           // waves per workgroup, splitting the image lines
+constexpr int kAdjK = 64 * kAdjDPL;     // detectors per workgroup
 
 // One lane's sum over its detector k, wave walking lines wave, wave + 4, ...
 // Per line the wave first evaluates every pixel of the span its 64 detectors
@@ -752,13 +754,15 @@ constexpr int kAdjK = 64;
 // along y (rows of imgT).  [SpLo, SpHi]: the span's bounds before the line's
 // term, [Alo, Ahi]: the lane's run's.
 template <bool AX>
-__device__ __forceinline__ float adj_gather(const BpParams& p, const int2* inside, const float2* origin,
-                                            const float* __restrict__ src, float c, float s, int k, float B,
-                                            float Alo, float Ahi, float SpLo, float SpHi, int ltx, int lty, int wave,
-                                            int lane, int* bk, float2* bv, int cap)
+__device__ __forceinline__ void adj_gather(const BpParams& p, const int2* inside, const float2* origin,
+                                           const float* __restrict__ src, float c, float s, const int (&k)[kAdjDPL],
+                                           float B, const float (&Alo)[kAdjDPL], const float (&Ahi)[kAdjDPL],
+                                           float SpLo, float SpHi, int ltx, int lty, int wave, int lane, int* bk,
+                                           float2* bv, int cap, float (&acc)[kAdjDPL])
 {
-    float acc = 0.0f;
-    for (int u = wave; u < p.N; u += 4) {
+#pragma unroll
+    for (int j = 0; j < kAdjDPL; j++) acc[j] = 0.0f;
+    for (int u = wave; u < p.N; u += kAdjWaves) {
         const float lc = ((float)u - p.center) * p.scale;
         const int2 in_u = inside[u];
         const int W0 = max(in_u.x, (int)floorf(__builtin_fmaf(-B, lc, SpLo)));
@@ -781,21 +785,23 @@ __device__ __forceinline__ float adj_gather(const BpParams& p, const int2* insid
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int r0 = max(W0, (int)floorf(__builtin_fmaf(-B, lc, Alo)));
-        const int r1 = min(W1, (int)ceilf(__builtin_fmaf(-B, lc, Ahi)));
-        for (int w = r0; w <= r1; w++) {
-            const int kk = bk[w - W0];
-            const float2 e = bv[w - W0];
-            acc += kk == k ? e.x : (kk + 1 == k ? e.y : 0.0f);
+#pragma unroll
+        for (int j = 0; j < kAdjDPL; j++) {
+            const int r0 = max(W0, (int)floorf(__builtin_fmaf(-B, lc, Alo[j])));
+            const int r1 = min(W1, (int)ceilf(__builtin_fmaf(-B, lc, Ahi[j])));
+            for (int w = r0; w <= r1; w++) {
+                const int kk = bk[w - W0];
+                const float2 e = bv[w - W0];
+                acc[j] += kk == k[j] ? e.x : (kk + 1 == k[j] ? e.y : 0.0f);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    return acc;
 }
 
-__global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict__ img,
+__global__ void __launch_bounds__(64 * kAdjWaves) bp_adjoint_kernel(const float* __restrict__ img,
                                                          const float* __restrict__ imgT, float* __restrict__ sino,
                                                          const float* __restrict__ cosT,
                                                          const float* __restrict__ sinT, BpParams p, int ltx, int lty,
@@ -804,11 +810,14 @@ __global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict
     extern __shared__ __attribute__((aligned(16))) char smem[];
     int2* inside = reinterpret_cast<int2*>(smem);                     // [N] per line: [lo, hi] inside the mask
     float2* origin = reinterpret_cast<float2*>(inside + p.N);         // per tile: (-T, bits of (int)T + cI)
-    float* part = reinterpret_cast<float*>(origin + p.tilesX * tilesY);   // [4 waves][kAdjK]
-    int* bk = reinterpret_cast<int*>(part + 4 * kAdjK);                     // [4 waves][cap]
-    float2* bv = reinterpret_cast<float2*>(bk + 4 * cap);                   // [4 waves][cap]
+    float* part = reinterpret_cast<float*>(origin + p.tilesX * tilesY);   // [waves][kAdjK]
+    int* bk = reinterpret_cast<int*>(part + kAdjWaves * kAdjK);             // [waves][cap]
+    float2* bv = reinterpret_cast<float2*>(bk + kAdjWaves * cap);           // [waves][cap]
     const int a = blockIdx.y, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int k = blockIdx.x * kAdjK + lane;
+    const int k0 = blockIdx.x * kAdjK;
+    int k[kAdjDPL];
+#pragma unroll
+    for (int j = 0; j < kAdjDPL; j++) k[j] = k0 + 64 * j + lane;
     const float c = cosT[a], s = sinT[a];
     const int TX = 1 << ltx, TY = 1 << lty;
     for (int t = threadIdx.x; t < p.tilesX * tilesY; t += blockDim.x) {
@@ -855,25 +864,34 @@ __global__ void __launch_bounds__(256) bp_adjoint_kernel(const float* __restrict
     // w = (t - up ss) / (cc scale) + center
     const float cc = alongX ? c : s, ss = alongX ? s : c;
     const float r = 1.0f / (cc * p.scale), B = ss * r;
-    const float A0 = (float)(k - p.cI - 1) * r + p.center, A1 = (float)(k - p.cI + 1) * r + p.center;
-    const float Alo = fminf(A0, A1) - 0.015625f, Ahi = fmaxf(A0, A1) + 0.015625f;
+    float Alo[kAdjDPL], Ahi[kAdjDPL];
+#pragma unroll
+    for (int j = 0; j < kAdjDPL; j++) {
+        const float A0 = (float)(k[j] - p.cI - 1) * r + p.center, A1 = (float)(k[j] - p.cI + 1) * r + p.center;
+        Alo[j] = fminf(A0, A1) - 0.015625f;
+        Ahi[j] = fmaxf(A0, A1) + 0.015625f;
+    }
     // the span of the workgroup's detectors k0 .. k0 + kAdjK - 1
-    const int k0 = blockIdx.x * kAdjK;
     const float S0 = (float)(k0 - p.cI - 1) * r + p.center, S1 = (float)(k0 + kAdjK - p.cI) * r + p.center;
     const float SpLo = fminf(S0, S1) - 0.015625f, SpHi = fmaxf(S0, S1) + 0.015625f;
     int* wbk = bk + wave * cap;
     float2* wbv = bv + wave * cap;
-    const float acc =
-        alongX ? adj_gather<true>(p, inside, origin, img, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk,
-                                  wbv, cap)
-               : adj_gather<false>(p, inside, origin, imgT, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk,
-                                   wbv, cap);
-    part[wave * kAdjK + lane] = acc;
+    float acc[kAdjDPL];
+    if (alongX)
+        adj_gather<true>(p, inside, origin, img, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk, wbv, cap,
+                         acc);
+    else
+        adj_gather<false>(p, inside, origin, imgT, c, s, k, B, Alo, Ahi, SpLo, SpHi, ltx, lty, wave, lane, wbk, wbv,
+                          cap, acc);
+#pragma unroll
+    for (int j = 0; j < kAdjDPL; j++) part[wave * kAdjK + 64 * j + lane] = acc[j];
     __syncthreads();
-    if (wave == 0 && k < p.D) {
-        const float sum = ((part[lane] + part[kAdjK + lane]) + part[2 * kAdjK + lane]) + part[3 * kAdjK + lane];
-        sino[(long)a * p.D + k] = sum * p.norm * p.out_scale;
-    }
+    for (int t = threadIdx.x; t < kAdjK; t += blockDim.x)
+        if (k0 + t < p.D) {
+            float sum = part[t];
+            for (int w = 1; w < kAdjWaves; w++) sum += part[w * kAdjK + t];
+            sino[(long)a * p.D + k0 + t] = sum * p.norm * p.out_scale;
+        }
 }
 
 // Standalone Masking (Masking.cu:113-144): optional min-max normalisation
@@ -1173,8 +1191,8 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
         // a line's span for kAdjK detectors: (kAdjK + 2) / (|cos or sin| scale)
         // pixels with |cos or sin| >= 1/sqrt(2), plus slack and rounding
         const int cap = std::min(p.N, (int)std::ceil((kAdjK + 2) * 1.41422 / p.scale) + 6);
-        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + 4 * kAdjK * sizeof(float) +
-                           4 * (size_t)cap * 12;
+        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + kAdjWaves * kAdjK * sizeof(float) +
+                           (size_t)kAdjWaves * cap * 12;
         if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
         DeviceGuard dg(h->device);
         hipStream_t s = as_stream(stream);
@@ -1187,7 +1205,7 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
                            h->imgT.ptr, p.N);
         check_launch("transpose_kernel");
         dim3 grid(ceil_div(p.D, kAdjK), p.P);
-        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(256), lds, s, img, h->imgT.ptr, sino, h->cosT.ptr,
+        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64 * kAdjWaves), lds, s, img, h->imgT.ptr, sino, h->cosT.ptr,
                            h->sinT.ptr, p, ltx, lty, tilesY, cap);
         check_launch("bp_adjoint_kernel");
     });
