@@ -513,6 +513,172 @@ __global__ void __launch_bounds__(512) dft_gemm_split_kernel(const float* __rest
     }
 }
 
+// The same product with the circulant read as what it is: B[n][k] =
+// c[(k - n) mod D] for one D-entry vector, split hi / lo.  The workgroup
+// keeps 8 copies of each vector in LDS, copy s shifted by s entries
+// (copy_s[i] = c[(i + s) mod D]), so the 8 consecutive k of a B fragment,
+// c[j0 .. j0 + 7] with j0 = (k - n) mod D, are one aligned 16-byte read of
+// copy (j0 mod 8): no B tile is loaded or staged.  That leaves LDS for two A
+// tiles: tile t + 1 is staged while tile t is multiplied, and the loads of
+// tile t + 2 are issued before, into the other of two register sets, so a
+// tile's loads have two tiles of MFMAs to arrive; one barrier per tile.
+// Each 32 x 32 block accumulates hi*hi, lo*hi, hi*lo in the order of
+// dft_gemm_split_kernel: the results are the same bits.  Workgroups take
+// their tiles in an XCD-aware order (the N tiles of a row block share an
+// XCD's L2 for their A rows).  kCircMinD <= D <= kCircMaxD; dynamic LDS
+// 2 * 8 * circ_pitch(D) halves.
+// LDS: 40 KB of A tiles + 32 * circ_pitch(D) bytes of copies <= 64 KB
+constexpr int kCircMinD = 128, kCircMaxD = 704;
+__host__ __device__ constexpr int circ_pitch(int D) { return ((D + 8 + 7) / 8) * 8 + 8; }
+
+__global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restrict__ X, int M, int D,
+                                                            const _Float16* __restrict__ ch,
+                                                            const _Float16* __restrict__ cl, float sx,
+                                                            float inv_scale, GemmOut o)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 Ah[2][GB_M][GB_PITCH];
+    __shared__ __attribute__((aligned(16))) _Float16 Al[2][GB_M][GB_PITCH];
+    extern __shared__ __attribute__((aligned(16))) _Float16 cs[];   // [2 (hi, lo)][8 shifts][pitch]
+    const int pitch = circ_pitch(D);
+
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    int mb, nb;
+    {
+        // workgroups are dealt round-robin to the 8 XCDs: the N tiles of one
+        // row block take ids 8 apart (groups of 8 row blocks; the rest plain)
+        const int NB = gridDim.x, L = blockIdx.x + NB * blockIdx.y, full = (gridDim.y / 8) * 8 * NB;
+        if (L < full) {
+            mb = (L / (8 * NB)) * 8 + L % 8;
+            nb = (L / 8) % NB;
+        } else {
+            mb = L / NB;
+            nb = L % NB;
+        }
+    }
+    const int m0 = mb * GB_M, n0 = nb * GB_N;
+    const int wm = (wave / 2) * 32, wn = (wave % 2) * 64;
+    const bool x4 = o.x4 != nullptr;
+    const int groups = (o.slices + 3) / 4;
+    const int rows = x4 ? groups * o.P * 4 : M;
+
+    for (int e = tid; e < 8 * pitch; e += blockDim.x) {
+        const int sft = e / pitch, i = e - sft * pitch;
+        const int j = (i + sft) % D;
+        cs[e] = ch[j];
+        cs[8 * pitch + e] = cl[j];
+    }
+
+    // this thread's staging slot: row sr, 8 consecutive k from sc8
+    const int sr = tid / 4, sc8 = (tid % 4) * 8;
+    long xrow = -1;
+    {
+        const int m = m0 + sr;
+        if (m < rows) {
+            if (x4) {
+                const int b = m & 3, ga = m >> 2, g = ga / o.P, a = ga - g * o.P, sl = g * 4 + b;
+                if (sl < o.slices) xrow = (long)sl * o.P + a;
+            } else {
+                xrow = m;
+            }
+        }
+    }
+    const bool vec = (D % 8) == 0;
+
+    float16_t acc0 = {}, acc1 = {};
+    auto load = [&](int k0, float (&av)[8]) {
+        const int gk = k0 + sc8;
+        if (xrow >= 0 && vec && gk + 8 <= D) {
+            const float4* src = reinterpret_cast<const float4*>(X + xrow * D + gk);
+            const float4 a = src[0], b = src[1];
+            av[0] = a.x; av[1] = a.y; av[2] = a.z; av[3] = a.w;
+            av[4] = b.x; av[5] = b.y; av[6] = b.z; av[7] = b.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; u++) av[u] = (xrow >= 0 && gk + u < D) ? X[xrow * D + gk + u] : 0.0f;
+        }
+    };
+    auto stage = [&](const float (&av)[8], int buf) {
+        half8_t hi, lo;
+        split8(av, sx, hi, lo);
+        *reinterpret_cast<half8_t*>(&Ah[buf][sr][sc8]) = hi;
+        *reinterpret_cast<half8_t*>(&Al[buf][sr][sc8]) = lo;
+    };
+    // B fragment of column n at k .. k + 7: j0 = (k - n) mod D by one
+    // correction each way (n is reduced mod D first: columns past D only feed
+    // outputs that are not stored; k < D + GB_K <= 2D)
+    auto bfrag = [&](int n, int k, int hl) {
+        int j0 = k - (n >= D ? n - D : n);
+        j0 += j0 < 0 ? D : 0;
+        j0 -= j0 >= D ? D : 0;
+        return *reinterpret_cast<const half8_t*>(&cs[(hl * 8 + (j0 & 7)) * pitch + (j0 & ~7)]);
+    };
+    auto mma = [&](int buf, int k0) {
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 16) {
+            const int r = lane % 32, kq = kk + (lane / 32) * 8;
+            const half8_t ah = *reinterpret_cast<const half8_t*>(&Ah[buf][wm + r][kq]);
+            const half8_t al = *reinterpret_cast<const half8_t*>(&Al[buf][wm + r][kq]);
+            const int n = n0 + wn + r, k = k0 + kq;
+            const half8_t bh0 = bfrag(n, k, 0), bl0 = bfrag(n, k, 1);
+            const half8_t bh1 = bfrag(n + 32, k, 0), bl1 = bfrag(n + 32, k, 1);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, acc1, 0, 0, 0);
+        }
+    };
+    // tiles t = 0 .. T-1 of GB_K; registers r0 hold even tiles, r1 odd ones
+    const int T = (D + GB_K - 1) / GB_K;
+    float r0[8], r1[8];
+    load(0, r0);
+    if (T > 1) load(GB_K, r1);
+    stage(r0, 0);
+    __syncthreads();
+    for (int t = 0; t < T; t += 2) {
+        // even tile t in buffer 0
+        if (t + 2 < T) load((t + 2) * GB_K, r0);
+        mma(0, t * GB_K);
+        if (t + 1 < T) stage(r1, 1);
+        __syncthreads();
+        if (t + 1 >= T) break;
+        // odd tile t + 1 in buffer 1
+        if (t + 3 < T) load((t + 3) * GB_K, r1);
+        mma(1, (t + 1) * GB_K);
+        if (t + 2 < T) stage(r0, 0);
+        __syncthreads();
+    }
+    // D layout: lane -> column lane%32; value v -> row 8*(v/4) + 4*(lane/32) + v%4
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const float16_t& acc = j == 0 ? acc0 : acc1;
+        const int col = n0 + wn + 32 * j + (lane % 32);
+        if (col >= D) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int rb = m0 + wm + 8 * q + 4 * (lane / 32);
+            if (x4) {
+                if (rb >= rows) continue;
+                uint32_t h[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) h[b] = float_to_half_bits(acc[4 * q + b] * inv_scale * o.out_scale);
+                o.x4[(long)(rb >> 2) * D + col] = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int row = rb + b;
+                    if (row >= M) continue;
+                    const float y = acc[4 * q + b] * inv_scale;
+                    if (o.y32) o.y32[(long)row * D + col] = y;
+                    if (o.y16) o.y16[(long)row * D + col] = float_to_half_bits(y * o.out_scale);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace risa
 
 struct risa_filter_s {
@@ -523,7 +689,8 @@ struct risa_filter_s {
     risa::FftPlan plan{};
     risa::DeviceBuffer<float2> tw;        // exp(-2 pi i n / D)
     risa::DeviceBuffer<float> H;          // full symmetric response [D]
-    risa::DeviceBuffer<_Float16> Ch, Cl;  // split circulant (GEMM mode)
+    risa::DeviceBuffer<_Float16> Ch, Cl;    // split circulant as a D x D matrix (GEMM mode, other D)
+    risa::DeviceBuffer<_Float16> ch1, cl1;  // split circulant vector c[d] (GEMM mode, 128 <= D <= 704)
     float sc = 1.0f;                      // circulant scale
     risa::DeviceBuffer<float> scratch;    // grad partial sums
     risa::DeviceBuffer<float> tmp;        // adjoint / out-of-place staging
@@ -565,16 +732,26 @@ static void upload_response(risa_filter_s* h)
         for (double v : c) cmax = std::max(cmax, std::fabs(v));
         // scale so the largest entry sits near 2^10 (well inside FP16 range)
         h->sc = cmax > 0 ? (float)std::exp2(std::floor(std::log2(1024.0 / cmax))) : 1.0f;
-        std::vector<_Float16> ch((size_t)D * D), cl((size_t)D * D);
-        for (int r = 0; r < D; r++)
-            for (int col = 0; col < D; col++) {
-                const float v = (float)(c[(size_t)((col - r + D) % D)] * h->sc);
-                const _Float16 hi = (_Float16)v;
-                ch[(size_t)r * D + col] = hi;
-                cl[(size_t)r * D + col] = (_Float16)(v - (float)hi);
-            }
-        h->Ch.upload(ch);
-        h->Cl.upload(cl);
+        std::vector<_Float16> vh(D), vl(D);
+        for (int d = 0; d < D; d++) {
+            const float v = (float)(c[(size_t)d] * h->sc);
+            vh[d] = (_Float16)v;
+            vl[d] = (_Float16)(v - (float)vh[d]);
+        }
+        if (D >= kCircMinD && D <= kCircMaxD) {
+            h->ch1.upload(vh);
+            h->cl1.upload(vl);
+        } else {
+            // B[n][k] = c[(k - n) mod D]
+            std::vector<_Float16> ch((size_t)D * D), cl((size_t)D * D);
+            for (int r = 0; r < D; r++)
+                for (int col = 0; col < D; col++) {
+                    ch[(size_t)r * D + col] = vh[(size_t)((col - r + D) % D)];
+                    cl[(size_t)r * D + col] = vl[(size_t)((col - r + D) % D)];
+                }
+            h->Ch.upload(ch);
+            h->Cl.upload(cl);
+        }
     }
 }
 
@@ -600,8 +777,13 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
             h->tw.upload(tw);
             h->H.resize(D);
             if (mode == RISA_FILTER_MODE_GEMM_SPLIT) {
-                h->Ch.resize((size_t)D * D);
-                h->Cl.resize((size_t)D * D);
+                if (D >= kCircMinD && D <= kCircMaxD) {
+                    h->ch1.resize(D);
+                    h->cl1.resize(D);
+                } else {
+                    h->Ch.resize((size_t)D * D);
+                    h->Cl.resize((size_t)D * D);
+                }
             }
             h->taps = filter_taps(*g);
             upload_response(h);
@@ -657,6 +839,14 @@ static void launch_gemm(risa_filter_t h, const float* in, GemmOut o, long rows, 
     const float inv = 1.0f / (sx * h->sc);
     const long out_rows = o.x4 ? (long)((o.slices + 3) / 4) * o.P * 4 : rows;
     dim3 grid(ceil_div(D, GB_N), ceil_div(out_rows, GB_M));
+    if (D >= kCircMinD && D <= kCircMaxD) {
+        const size_t lds = (size_t)2 * 8 * circ_pitch(D) * sizeof(_Float16);
+        if (lds + sizeof(_Float16) * 2 * 2 * GB_M * GB_PITCH > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "circulant LDS");
+        hipLaunchKernelGGL(dft_gemm_circ_kernel, grid, dim3(512), lds, s, in, (int)rows, D, h->ch1.ptr, h->cl1.ptr, sx,
+                           inv, o);
+        check_launch("dft_gemm_circ_kernel");
+        return;
+    }
     hipLaunchKernelGGL(dft_gemm_split_kernel, grid, dim3(512), 0, s, in, (int)rows, D, h->Ch.ptr, h->Cl.ptr, sx, inv,
                        o);
     check_launch("dft_gemm_split_kernel");
