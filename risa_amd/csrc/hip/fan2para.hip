@@ -86,19 +86,23 @@ __device__ __forceinline__ float mix_rays(bool r1, bool r2, float z1, float z2)
 }
 
 __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fan, const uint4* __restrict__ table,
-                                                    const int4* __restrict__ boxes, float* __restrict__ par, int Df,
-                                                    int Pf, int D, int P, long table_plane_stride,
-                                                    long box_plane_stride, FrameGroups fg)
+                                                    const int4* __restrict__ boxes, const int2* __restrict__ order,
+                                                    float* __restrict__ par, int Df, int Pf, int D, int P,
+                                                    long table_plane_stride, long box_plane_stride, FrameGroups fg)
 {
     __shared__ float patch[4 * kPatchMax];
+    // workgroup x takes output block order[plane][x]: blocks with a slot that
+    // gathers from global memory first, so the slowest blocks start early
+    const int grp = blockIdx.y;
+    const int plane = fg.plane[grp];
+    const int nbx = (D + kRebinBlock - 1) / kRebinBlock;
+    const int2 ob = order[(long)plane * gridDim.x + blockIdx.x];
     // a wave covers an 8 x 8 sub-block, so global fallback gathers stay local too
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int i = blockIdx.x * kRebinBlock + (w & 1) * 8 + (l & 7);
-    const int j = blockIdx.y * kRebinBlock + (w >> 1) * 8 + (l >> 3);
-    const int grp = blockIdx.z;
+    const int i = ob.x * kRebinBlock + (w & 1) * 8 + (l & 7);
+    const int j = ob.y * kRebinBlock + (w >> 1) * 8 + (l >> 3);
     const bool inside = i < D && j < P;
-    const int plane = fg.plane[grp];
-    const int4* bx = boxes + plane * box_plane_stride + ((long)blockIdx.y * gridDim.x + blockIdx.x) * 4;
+    const int4* bx = boxes + plane * box_plane_stride + ((long)ob.y * nbx + ob.x) * 4;
     int4 box[4];
 #pragma unroll
     for (int sl = 0; sl < 4; sl++) box[sl] = bx[sl];
@@ -263,6 +267,7 @@ struct risa_fan2para_s {
     size_t plane_words = 0;
     risa::DeviceBuffer<int4> boxes;       // [planes][block rows][block cols][4 slots]
     long box_plane = 0;
+    risa::DeviceBuffer<int2> order;       // [planes][blocks]: (block col, block row), global-gather blocks first
 };
 
 using namespace risa;
@@ -331,6 +336,21 @@ extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int w
                 }
                 RISA_HIP_CHECK(hipMemcpy(h->boxes.ptr + k * h->box_plane, bx.data(), bx.size() * sizeof(int4),
                                          hipMemcpyHostToDevice));
+                {
+                    // launch order: blocks with a global-gather slot first
+                    const int nbx = ceil_div(D, kRebinBlock), nby = ceil_div(P, kRebinBlock);
+                    std::vector<int2> heavy, light;
+                    for (int by = 0; by < nby; by++)
+                        for (int bxi = 0; bxi < nbx; bxi++) {
+                            bool slow = false;
+                            for (int sl = 0; sl < 4; sl++) slow |= bx[((size_t)by * nbx + bxi) * 4 + sl].z < 0;
+                            (slow ? heavy : light).push_back(make_int2(bxi, by));
+                        }
+                    heavy.insert(heavy.end(), light.begin(), light.end());
+                    if (k == 0) h->order.resize(heavy.size() * g->numberOfPlanes);
+                    RISA_HIP_CHECK(hipMemcpy(h->order.ptr + (size_t)k * heavy.size(), heavy.data(),
+                                             heavy.size() * sizeof(int2), hipMemcpyHostToDevice));
+                }
                 if (weights == RISA_REBIN_WEIGHTS_FP64) {
                     RebinGeo rg{};
                     rg.P = P; rg.P2 = 2 * P; rg.D = D;
@@ -387,10 +407,10 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
                 pos += cnt;
             }
         }
-        dim3 grid(ceil_div(D, kRebinBlock), ceil_div(P, kRebinBlock), groups);
+        dim3 grid(ceil_div(D, kRebinBlock) * ceil_div(P, kRebinBlock), groups);
         hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, s, fan + (size_t)f0 * Df * Pf,
-                           reinterpret_cast<const uint4*>(h->table.ptr), h->boxes.ptr, par + (size_t)f0 * D * P,
-                           Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
+                           reinterpret_cast<const uint4*>(h->table.ptr), h->boxes.ptr, h->order.ptr,
+                           par + (size_t)f0 * D * P, Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
         check_launch("rebin_kernel");
     }
 }
