@@ -365,15 +365,22 @@ int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* plane
  * layout, one slot per frame). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces
- * the autotuner's pick).  Only before the chain's first graph run, else
- * RISA_ERR_UNSUPPORTED.  No reference counterpart. */
+ * the autotuner's pick).  A change synchronises the chain's device and
+ * releases the captured graphs; the next graph run captures anew.  No
+ * reference counterpart. */
 int risa_hip_chain_set_variant(risa_chain_t h, int variant);
 int risa_hip_chain_get_variant(risa_chain_t h, int* variant);
 /* Layout of the raw frames risa_hip_chain_run reads (risa_input_layout;
  * MODULE_MAJOR = the reference's H2D -> Reordering -> Attenuation, with the
- * reordering fused into the attenuation load).  Only before the chain's first
- * graph run, else RISA_ERR_UNSUPPORTED. */
+ * reordering fused into the attenuation load).  A change synchronises the
+ * device and releases the captured graphs, as above. */
 int risa_hip_chain_set_input_layout(risa_chain_t h, int layout);
+/* Graph executables the chain holds: one per (raw, img, frames, planes) call
+ * shape, at most 128 (the least recently replayed one is released, after a
+ * device synchronisation, to make room).  Tooling; no reference counterpart. */
+int risa_hip_chain_graph_count(risa_chain_t h, int* count);
+/* Synchronises the chain's device, then releases graphs, streams and
+ * buffers (safe while replays issued on any stream are still running). */
 int risa_hip_chain_destroy(risa_chain_t h);
 
 #ifdef __cplusplus

@@ -202,6 +202,7 @@ _SIGS = {
     "risa_hip_chain_set_variant": [P, C.c_int],
     "risa_hip_chain_get_variant": [P, C.POINTER(C.c_int)],
     "risa_hip_chain_set_input_layout": [P, C.c_int],
+    "risa_hip_chain_graph_count": [P, C.POINTER(C.c_int)],
     "risa_hip_chain_destroy": [P],
     # synthetic source (include/risa_synth.h)
     "risa_synth_phantom": [C.c_uint64, C.c_double, C.POINTER(SynthOpts), P, C.c_int],

@@ -1,8 +1,10 @@
 // CPU self-test of the pipeline framework and the stages' host side: no GPU
 // is touched.  Checks the queue bound and order, pool backpressure and
 // release, sentinel propagation through Source -> Stage -> Stage -> Sink,
-// the module-major reference file layout, and that stages reject a
-// configuration with a missing key.
+// the module-major reference file layout, that stages reject a
+// configuration with a missing key, and the copy stages' frame accounting
+// (lost indices, reconstruction rate) on a CPU pipeline whose source skips
+// indices.
 //
 //   glados_selftest <config>      exit 0 when every check passes
 #include <atomic>
@@ -161,6 +163,96 @@ void test_pipeline()
     expect(ok, "37 frames pass two stages in order and the sentinel ends every stage");
 }
 
+// Source whose indices skip every k-th value (frames lost before the GPU);
+// the stage in front counts them with H2D's counter, the one behind keeps
+// D2H's rate meter.
+class GappyLoader {
+public:
+    using manager_type = HostInt;
+    GappyLoader(int n, int k) : n_(n), k_(k) {}
+    IntImage loadImage()
+    {
+        if (emitted_ >= n_) return IntImage();
+        if (k_ > 0 && next_ % k_ == k_ - 1) ++next_;
+        IntImage img(4, (std::size_t)next_);
+        ++next_;
+        ++emitted_;
+        return img;
+    }
+
+private:
+    int n_, k_, next_ = 0, emitted_ = 0;
+};
+
+class CountLoss {
+public:
+    using input_type = IntImage;
+    using output_type = IntImage;
+    void process(input_type&& img)
+    {
+        if (img.valid()) loss.arrived(img.index());
+        q_.push(std::move(img));
+    }
+    output_type wait() { return q_.take(); }
+    risa::FrameLossCounter loss;
+
+private:
+    glados::Queue<IntImage> q_;
+};
+
+class MeterRate {
+public:
+    using input_type = IntImage;
+    using output_type = IntImage;
+    MeterRate() : rate(8) {}
+    void process(input_type&& img)
+    {
+        if (img.valid()) {
+            rate.arrived(img.index());
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        q_.push(std::move(img));
+    }
+    output_type wait() { return q_.take(); }
+    risa::RateMeter rate;
+
+private:
+    glados::Queue<IntImage> q_;
+};
+
+void test_accounting()
+{
+    using namespace glados::pipeline;
+    Collected got;
+    Pipeline p;
+    // 50 frames, every 5th index left out: indices 0..61 minus 4, 9, ..., 59
+    auto src = p.create<SourceStage<GappyLoader>>(50, 5);
+    auto loss = p.create<Stage<CountLoss>>();
+    auto rate = p.create<Stage<MeterRate>>();
+    auto sink = p.create<SinkStage<Collector>>("", "x", &got);
+    p.connect(src, loss);
+    p.connect(loss, rate);
+    p.connect(rate, sink);
+    p.run(src, loss, rate, sink);
+    p.wait();
+    expect(got.idx.size() == 50, "gappy source: 50 frames reach the sink");
+    expect(loss->loss.lost() == 12, "lost-frame count = the 12 skipped indices");
+    expect(loss->loss.last_index() == 61, "last index seen is 61");
+    const double pct = loss->loss.loss_percent();
+    expect(pct > 19.67 && pct < 19.68, "loss percentage = lost / last index * 100 (12/61)");
+    expect(loss->loss.worst_case_seconds() >= loss->loss.best_case_seconds(), "inter-arrival extremes ordered");
+    expect(rate->rate.images() == 50 && rate->rate.windows() == 6, "rate meter: 6 full windows of 8 images");
+    const double r = rate->rate.reconstruction_rate(), all = rate->rate.images_per_second();
+    expect(r > 0 && r < 5000.0 + 1e-9 && all > 0 && all < 5000.0 + 1e-9, "rates bounded by the 200 us per image");
+    // a gap-free stream from 0 loses nothing; out-of-order arrivals subtract
+    risa::FrameLossCounter c;
+    for (std::size_t i = 0; i < 10; i++) c.arrived(i);
+    expect(c.lost() == 0 && c.loss_percent() == 0.0, "indices 0..9 lose nothing");
+    c.arrived(15);
+    c.arrived(13);
+    expect(c.lost() == (15 - 9 - 1) + (13 - 15 - 1), "gap adds, out-of-order arrival subtracts (reference arithmetic)");
+}
+
 void test_module_files(const std::string& config)
 {
     risa_geom g = risa::read_geometry(config);
@@ -233,6 +325,7 @@ int main(int argc, char** argv)
     test_queue();
     test_pool();
     test_pipeline();
+    test_accounting();
     test_module_files(argv[1]);
     test_stage_config_errors(argv[1]);
     std::printf("glados_selftest: %d/%d checks passed\n", checks - failures, checks);

@@ -9,6 +9,13 @@
 // keeps the first few slices for checking.
 //
 //   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] [--tiff]
+//                 [--skip-every K]
+//
+// precision = "mixed" in the configuration wires FilterMixed and
+// BackprojectionMixed (FP16 filtered sinograms) in place of Filter and
+// Backprojection.
+// --skip-every K: the source leaves out every index i with i % K == K - 1
+// (frames lost before the GPU), which H2D's accounting counts.
 //
 // --module-major: the source holds ROFEX module-major frames (27 modules x
 // P_f x 16, D_f = 432) and a Reordering stage follows H2D, as
@@ -16,7 +23,8 @@
 // --tiff: the sink also hands every slice to risa::OfflineSaver, which writes
 // TIFF files as the config's output keys say (outputPath, mode, ...).
 //
-// Prints one JSON line: frames, seconds, slices/s, mean and max latency.
+// Prints one JSON line: frames, seconds, slices/s, mean and max latency, the
+// precision, H2D's lost-frame count and loss percentage, and D2H's rates.
 // --save writes K slices as records {int32 index, int32 frame, int32 plane,
 // float img[N*N]} (frame = the seeded frame id, see below).
 #include <algorithm>
@@ -51,8 +59,9 @@ class SyntheticLoader {
 public:
     using manager_type = glados::hip::HostMemoryManager<uint16_t>;
 
-    SyntheticLoader(const std::string& configFile, long frames, int unique, Provenance* prov, bool module_major)
-        : frames_(frames), prov_(prov)
+    SyntheticLoader(const std::string& configFile, long frames, int unique, Provenance* prov, bool module_major,
+                    long skip_every)
+        : frames_(frames), skip_every_(skip_every), prov_(prov)
     {
         g_ = risa::read_geometry(configFile);
         const int Pf = g_.numberOfFanProjections, Df = g_.numberOfFanDetectors;
@@ -90,7 +99,8 @@ public:
 
     glados::Image<manager_type> loadImage()
     {
-        if (next_ >= frames_) return glados::Image<manager_type>();
+        if (emitted_ >= frames_) return glados::Image<manager_type>();
+        if (skip_every_ > 0 && next_ % skip_every_ == skip_every_ - 1) ++next_;   // index lost upstream
         auto img = glados::MemoryPool<manager_type>::instance()->requestMemory(pool_);
         const auto id = ids_.at(img.data());
         img.setIdx((std::size_t)next_);
@@ -101,12 +111,13 @@ public:
             prov_->of_index[(std::size_t)next_] = id;
         }
         ++next_;
+        ++emitted_;
         return img;
     }
 
 private:
     risa_geom g_{};
-    long frames_ = 0, next_ = 0;
+    long frames_ = 0, next_ = 0, emitted_ = 0, skip_every_ = 0;
     int pool_ = -1;
     Provenance* prov_;
     std::map<const uint16_t*, std::pair<int, int>> ids_;
@@ -150,6 +161,76 @@ private:
     std::unique_ptr<risa::OfflineSaver> tiff_;
 };
 
+struct Options {
+    std::string config;
+    long frames = 200;
+    int unique = 8, keep = 0;
+    long skip_every = 0;
+    bool masking = false, module_major = false, tiff = false;
+    std::string save_path;
+};
+
+struct RunStats {
+    double secs = 0.0;
+    long long lost = 0;
+    double loss_percent = 0.0, d2h_rate = 0.0, d2h_images_per_s = 0.0;
+};
+
+// The reference's wiring (main.cpp:95-134) with the given Filter and
+// Backprojection stage types.
+template <class FilterT, class BackprojectionT>
+RunStats run_pipeline(const Options& o, Provenance& prov, SinkState& st)
+{
+    using namespace glados::pipeline;
+    using Source = SourceStage<SyntheticLoader>;
+    using Sink = SinkStage<StatsSaver>;
+    Pipeline pipeline;
+    auto source = pipeline.create<Source>(o.config, o.frames, o.unique, &prov, o.module_major, o.skip_every);
+    auto h2d = pipeline.create<Stage<risa::H2D>>(o.config);
+    auto reordering = o.module_major ? pipeline.create<Stage<risa::Reordering>>(o.config) : nullptr;
+    auto attenuation = pipeline.create<Stage<risa::Attenuation>>(o.config);
+    auto fan2para = pipeline.create<Stage<risa::Fan2Para>>(o.config);
+    auto filter = pipeline.create<Stage<FilterT>>(o.config);
+    auto backprojection = pipeline.create<Stage<BackprojectionT>>(o.config);
+    auto masking_stage = o.masking ? pipeline.create<Stage<risa::Masking>>(o.config) : nullptr;
+    auto d2h = pipeline.create<Stage<risa::D2H>>(o.config);
+    auto sink = pipeline.create<Sink>("", "IMG", &st, o.tiff ? o.config : std::string());
+
+    pipeline.connect(source, h2d);
+    if (reordering) {
+        pipeline.connect(h2d, reordering);
+        pipeline.connect(reordering, attenuation);
+    } else {
+        pipeline.connect(h2d, attenuation);
+    }
+    pipeline.connect(attenuation, fan2para);
+    pipeline.connect(fan2para, filter);
+    pipeline.connect(filter, backprojection);
+    if (masking_stage) {
+        pipeline.connect(backprojection, masking_stage);
+        pipeline.connect(masking_stage, d2h);
+    } else {
+        pipeline.connect(backprojection, d2h);
+    }
+    pipeline.connect(d2h, sink);
+
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    pipeline.run(source, h2d);
+    if (reordering) pipeline.run(reordering);
+    pipeline.run(attenuation, fan2para, filter, backprojection);
+    if (masking_stage) pipeline.run(masking_stage);
+    pipeline.run(d2h, sink);
+    pipeline.wait();
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    RunStats r;
+    r.secs = std::chrono::duration<double>(t1 - t0).count();
+    r.lost = h2d->lostFrames();
+    r.loss_percent = h2d->lossPercent();
+    r.d2h_rate = d2h->reconstructionRate();
+    r.d2h_images_per_s = d2h->imagesPerSecond();
+    return r;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -157,82 +238,43 @@ int main(int argc, char** argv)
     if (argc < 2) {
         std::fprintf(stderr,
                      "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] "
-                     "[--tiff]\n",
+                     "[--tiff] [--skip-every K]\n",
                      argv[0]);
         return 2;
     }
-    const std::string config = argv[1];
-    long frames = 200;
-    int unique = 8, keep = 0;
-    bool masking = false, module_major = false, tiff = false;
-    std::string save_path;
+    Options o;
+    o.config = argv[1];
     for (int i = 2; i < argc; i++) {
         const std::string a = argv[i];
-        if (a == "--frames" && i + 1 < argc) frames = std::atol(argv[++i]);
-        else if (a == "--unique" && i + 1 < argc) unique = std::atoi(argv[++i]);
+        if (a == "--frames" && i + 1 < argc) o.frames = std::atol(argv[++i]);
+        else if (a == "--unique" && i + 1 < argc) o.unique = std::atoi(argv[++i]);
         else if (a == "--save" && i + 2 < argc) {
-            keep = std::atoi(argv[++i]);
-            save_path = argv[++i];
-        } else if (a == "--masking") masking = true;
-        else if (a == "--module-major") module_major = true;
-        else if (a == "--tiff") tiff = true;
+            o.keep = std::atoi(argv[++i]);
+            o.save_path = argv[++i];
+        } else if (a == "--masking") o.masking = true;
+        else if (a == "--module-major") o.module_major = true;
+        else if (a == "--tiff") o.tiff = true;
+        else if (a == "--skip-every" && i + 1 < argc) o.skip_every = std::atol(argv[++i]);
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
     }
-
-    using namespace glados::pipeline;
-    using Source = SourceStage<SyntheticLoader>;
-    using Sink = SinkStage<StatsSaver>;
+    if (o.skip_every == 1) {
+        std::fprintf(stderr, "--skip-every needs K >= 2\n");
+        return 2;
+    }
 
     Provenance prov;
     SinkState st;
-    st.keep = keep;
+    st.keep = o.keep;
     try {
-        Pipeline pipeline;
-        auto source = pipeline.create<Source>(config, frames, unique, &prov, module_major);
-        auto h2d = pipeline.create<Stage<risa::H2D>>(config);
-        auto reordering = module_major ? pipeline.create<Stage<risa::Reordering>>(config) : nullptr;
-        auto attenuation = pipeline.create<Stage<risa::Attenuation>>(config);
-        auto fan2para = pipeline.create<Stage<risa::Fan2Para>>(config);
-        auto filter = pipeline.create<Stage<risa::Filter>>(config);
-        auto backprojection = pipeline.create<Stage<risa::Backprojection>>(config);
-        auto masking_stage = masking ? pipeline.create<Stage<risa::Masking>>(config) : nullptr;
-        auto d2h = pipeline.create<Stage<risa::D2H>>(config);
-        auto sink = pipeline.create<Sink>("", "IMG", &st, tiff ? config : std::string());
-
-        pipeline.connect(source, h2d);
-        if (reordering) {
-            pipeline.connect(h2d, reordering);
-            pipeline.connect(reordering, attenuation);
-        } else {
-            pipeline.connect(h2d, attenuation);
-        }
-        pipeline.connect(attenuation, fan2para);
-        pipeline.connect(fan2para, filter);
-        pipeline.connect(filter, backprojection);
-        if (masking_stage) {
-            pipeline.connect(backprojection, masking_stage);
-            pipeline.connect(masking_stage, d2h);
-        } else {
-            pipeline.connect(backprojection, d2h);
-        }
-        pipeline.connect(d2h, sink);
-
-        const auto t0 = std::chrono::high_resolution_clock::now();
-        pipeline.run(source, h2d);
-        if (reordering) pipeline.run(reordering);
-        pipeline.run(attenuation, fan2para, filter, backprojection);
-        if (masking_stage) pipeline.run(masking_stage);
-        pipeline.run(d2h, sink);
-        pipeline.wait();
-        const auto t1 = std::chrono::high_resolution_clock::now();
-        const double secs = std::chrono::duration<double>(t1 - t0).count();
-
-        if (!save_path.empty()) {
-            std::FILE* f = std::fopen(save_path.c_str(), "wb");
-            if (!f) throw std::runtime_error("cannot open " + save_path);
+        const bool mixed = risa::mixed_precision(o.config);
+        const RunStats r = mixed ? run_pipeline<risa::FilterMixed, risa::BackprojectionMixed>(o, prov, st)
+                                 : run_pipeline<risa::Filter, risa::Backprojection>(o, prov, st);
+        if (!o.save_path.empty()) {
+            std::FILE* f = std::fopen(o.save_path.c_str(), "wb");
+            if (!f) throw std::runtime_error("cannot open " + o.save_path);
             for (auto& kv : st.kept) {
                 const auto id = prov.of_index.at(kv.first);
                 const int32_t hdr[3] = {(int32_t)kv.first, id.first, id.second};
@@ -242,11 +284,13 @@ int main(int argc, char** argv)
             std::fclose(f);
         }
         std::printf("{\"frames\": %ld, \"seconds\": %.4f, \"slices_per_s\": %.1f, \"latency_ms_mean\": %.3f, "
-                    "\"latency_ms_max\": %.3f, \"saved\": %zu}\n",
-                    st.count, secs, st.count / secs, st.count ? st.lat_sum / st.count : 0.0, st.lat_max,
-                    st.kept.size());
-        if (st.count != frames) {
-            std::fprintf(stderr, "expected %ld slices, got %ld\n", frames, st.count);
+                    "\"latency_ms_max\": %.3f, \"saved\": %zu, \"precision\": \"%s\", \"lost_frames\": %lld, "
+                    "\"loss_percent\": %.4f, \"d2h_reconstruction_rate\": %.1f, \"d2h_images_per_s\": %.1f}\n",
+                    st.count, r.secs, st.count / r.secs, st.count ? st.lat_sum / st.count : 0.0, st.lat_max,
+                    st.kept.size(), mixed ? "mixed" : "fp32", r.lost, r.loss_percent, r.d2h_rate,
+                    r.d2h_images_per_s);
+        if (st.count != o.frames) {
+            std::fprintf(stderr, "expected %ld slices, got %ld\n", o.frames, st.count);
             return 1;
         }
     } catch (const std::exception& e) {

@@ -587,5 +587,8 @@ extern "C" int risa_hip_reorder_run(const uint16_t* raw, int frames, int Pf, int
 
 extern "C" int risa_hip_attenuation_destroy(risa_attenuation_t h)
 {
-    return guarded("risa_hip_attenuation_destroy", [&] { delete h; });
+    return guarded("risa_hip_attenuation_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }

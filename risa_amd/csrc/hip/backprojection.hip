@@ -1015,9 +1015,15 @@ struct risa_bp_s {
     int variant = 0;
     int variant_f16 = 0;
     bool fuse_mask = false;
-    risa::DeviceBuffer<uint2> x4;   // staging for risa_hip_bp_run_f16's plain layout
-    risa::DeviceBuffer<float> imgT;  // transposed image for risa_hip_bp_adjoint_run
+    risa::DeviceBuffer<uint2> x4;   // staging for risa_hip_bp_run_f16's plain layout [kF16StageGroups][P][D]
+    risa::DeviceBuffer<float> imgT;  // transposed image for risa_hip_bp_adjoint_run [N][N]
 };
+
+namespace risa {
+// risa_hip_bp_run_f16 converts and backprojects up to this many groups of 4
+// slices per pass through the staging buffer allocated at create.
+constexpr int kF16StageGroups = 4;
+}  // namespace risa
 
 using namespace risa;
 
@@ -1079,6 +1085,10 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
             h->sinT.resize(h->lut.sinT.size());
             h->sinT.upload(h->lut.sinT);
             h->fuse_mask = g->fusedMasking && !g->normalization;
+            // run-call scratch, sized here so run calls allocate nothing
+            const size_t plane = (size_t)g->numberOfParallelProjections * g->numberOfParallelDetectors;
+            h->x4.resize((size_t)kF16StageGroups * plane);
+            h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);
         } catch (...) {
             delete h;
             throw;
@@ -1161,17 +1171,20 @@ extern "C" int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_s
     return guarded("risa_hip_bp_run_f16", [&] {
         if (!h || !sino || !img || slices < 0 || !(in_scale > 0)) fail(RISA_ERR_ARG, "bad argument");
         if (!slices) return;
-        DeviceGuard dg(h->device);
         hipStream_t s = as_stream(stream);
         const long plane = (long)h->g.numberOfParallelProjections * h->g.numberOfParallelDetectors;
-        const size_t need = (size_t)ceil_div(slices, kF16Group) * plane;
-        if (h->x4.count < need) {
-            RISA_HIP_CHECK(hipStreamSynchronize(s));
-            h->x4.resize(need);
+        // passes of up to kF16StageGroups * 4 slices through the staging
+        // buffer; stream order keeps a pass's conversion behind the previous
+        // pass's backprojection
+        const int per = kF16StageGroups * kF16Group;
+        for (int s0 = 0; s0 < slices; s0 += per) {
+            const int n = std::min(per, slices - s0);
+            hipLaunchKernelGGL(f16_to_x4_kernel, dim3(1024), dim3(256), 0, s, sino + (size_t)s0 * plane, h->x4.ptr, n,
+                               plane);
+            check_launch("f16_to_x4_kernel");
+            launch_f16x4(h, h->variant_f16, h->x4.ptr, in_scale, n, img + (size_t)s0 * h->g.numberOfPixels *
+                         h->g.numberOfPixels, s);
         }
-        hipLaunchKernelGGL(f16_to_x4_kernel, dim3(1024), dim3(256), 0, s, sino, h->x4.ptr, slices, plane);
-        check_launch("f16_to_x4_kernel");
-        launch_f16x4(h, h->variant_f16, h->x4.ptr, in_scale, slices, img, s);
     });
 }
 
@@ -1194,13 +1207,7 @@ extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sin
         const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + kAdjWaves * kAdjK * sizeof(float) +
                            (size_t)kAdjWaves * cap * 12;
         if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
-        DeviceGuard dg(h->device);
         hipStream_t s = as_stream(stream);
-        const size_t need = (size_t)p.N * p.N;
-        if (h->imgT.count < need) {
-            RISA_HIP_CHECK(hipStreamSynchronize(s));
-            h->imgT.resize(need);
-        }
         hipLaunchKernelGGL(transpose_kernel, dim3(ceil_div(p.N, 32), ceil_div(p.N, 32)), dim3(32, 8), 0, s, img,
                            h->imgT.ptr, p.N);
         check_launch("transpose_kernel");
@@ -1329,7 +1336,10 @@ extern "C" int risa_hip_bp_autotune(risa_bp_t h, void* stream)
 
 extern "C" int risa_hip_bp_destroy(risa_bp_t h)
 {
-    return guarded("risa_hip_bp_destroy", [&] { delete h; });
+    return guarded("risa_hip_bp_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }
 
 // ---------------------------------------------------------------- Masking --
@@ -1381,7 +1391,10 @@ extern "C" int risa_hip_mask_run(risa_mask_t h, float* img, void* stream)
 
 extern "C" int risa_hip_mask_destroy(risa_mask_t h)
 {
-    return guarded("risa_hip_mask_destroy", [&] { delete h; });
+    return guarded("risa_hip_mask_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }
 
 // ------------------------------------------- persistent uneven batches (C3) --
@@ -1597,5 +1610,8 @@ extern "C" int risa_hip_bp_batched_autotune(risa_bp_batched_t h, void* stream)
 
 extern "C" int risa_hip_bp_batched_destroy(risa_bp_batched_t h)
 {
-    return guarded("risa_hip_bp_batched_destroy", [&] { delete h; });
+    return guarded("risa_hip_bp_batched_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }

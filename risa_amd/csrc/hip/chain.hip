@@ -26,6 +26,15 @@ constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 *
 constexpr int kPlainLanes = 4;            // frames in flight in PLAIN mode
 }  // namespace
 
+// Graph executables a chain keeps (one per buffer set and call shape); the
+// least recently replayed one is released beyond this.
+constexpr size_t kMaxGraphs = 128;
+
+struct ChainGraph {
+    hipGraphExec_t exec = nullptr;
+    unsigned long long last_use = 0;
+};
+
 struct risa_chain_s {
     int device = 0;
     int mode = 0;
@@ -40,10 +49,30 @@ struct risa_chain_s {
     risa::DeviceBuffer<uint16_t> f16_buf;
     std::vector<hipStream_t> lane_streams;
     std::vector<hipEvent_t> lane_events;   // [0] fork, [1..lanes] joins
-    std::map<std::tuple<const void*, const void*, int, std::vector<int32_t>>, hipGraphExec_t> graphs;
+    std::map<std::tuple<const void*, const void*, int, std::vector<int32_t>>, ChainGraph> graphs;
+    unsigned long long uses = 0;
+
+    // A graph executable, event or stream must not be released while work
+    // that uses it may still run: hipGraphLaunch returns before the replay
+    // ends, and the caller's stream is not known here, so the whole device
+    // is drained first.
+    void drain() const
+    {
+        int prev = -1;
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        if (prev != device) (void)hipSetDevice(device);
+        (void)hipDeviceSynchronize();
+        if (prev != device) (void)hipSetDevice(prev);
+    }
+    void release_graphs()
+    {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
+        graphs.clear();
+    }
     ~risa_chain_s()
     {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        drain();
+        release_graphs();
         for (hipEvent_t e : lane_events) (void)hipEventDestroy(e);
         for (hipStream_t s : lane_streams) (void)hipStreamDestroy(s);
         if (bp) risa_hip_bp_destroy(bp);
@@ -166,22 +195,35 @@ extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int
         auto key = std::make_tuple((const void*)raw, (const void*)img, frames, std::vector<int32_t>(planes, planes + frames));
         auto it = h->graphs.find(key);
         if (it == h->graphs.end()) {
+            if (h->graphs.size() >= kMaxGraphs) {
+                // release the least recently replayed executable, once no
+                // replay of it can still be running
+                auto lru = h->graphs.begin();
+                for (auto jt = h->graphs.begin(); jt != h->graphs.end(); ++jt)
+                    if (jt->second.last_use < lru->second.last_use) lru = jt;
+                DeviceGuard dg(h->device);
+                RISA_HIP_CHECK(hipDeviceSynchronize());
+                RISA_HIP_CHECK(hipGraphExecDestroy(lru->second.exec));
+                h->graphs.erase(lru);
+            }
             hipGraph_t graph;
             RISA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
                 enqueue(h, raw, planes, frames, img, s);
             } catch (...) {
-                hipGraph_t dummy;
-                (void)hipStreamEndCapture(s, &dummy);
+                hipGraph_t dummy = nullptr;
+                if (hipStreamEndCapture(s, &dummy) == hipSuccess && dummy) (void)hipGraphDestroy(dummy);
                 throw;
             }
             RISA_HIP_CHECK(hipStreamEndCapture(s, &graph));
             hipGraphExec_t exec;
-            RISA_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            it = h->graphs.emplace(key, exec).first;
+            RISA_HIP_CHECK(ie);
+            it = h->graphs.emplace(key, ChainGraph{exec, 0}).first;
         }
-        RISA_HIP_CHECK(hipGraphLaunch(it->second, s));
+        it->second.last_use = ++h->uses;
+        RISA_HIP_CHECK(hipGraphLaunch(it->second.exec, s));
     });
 }
 
@@ -198,12 +240,18 @@ extern "C" int risa_hip_chain_intermediate(risa_chain_t h, int which, const void
     });
 }
 
+// Captured graphs hold the launches of the layout and variant they were
+// captured with: a change drains the device, releases them, and the next
+// graph run captures anew.
 extern "C" int risa_hip_chain_set_input_layout(risa_chain_t h, int layout)
 {
     return guarded("risa_hip_chain_set_input_layout", [&] {
         if (!h) fail(RISA_ERR_ARG, "null handle");
-        // captured graphs hold the attenuation launches of the layout they saw
-        if (!h->graphs.empty()) fail(RISA_ERR_UNSUPPORTED, "set the chain's input layout before its first graph run");
+        int cur = 0;
+        ok(risa_hip_attenuation_get_input_layout(h->att, &cur));
+        if (cur == layout) return;
+        h->drain();
+        h->release_graphs();
         ok(risa_hip_attenuation_set_input_layout(h->att, layout));
     });
 }
@@ -212,10 +260,21 @@ extern "C" int risa_hip_chain_set_variant(risa_chain_t h, int variant)
 {
     return guarded("risa_hip_chain_set_variant", [&] {
         if (!h) fail(RISA_ERR_ARG, "null handle");
-        // captured graphs hold the launches of the variant they were captured
-        // with; they are released only with the chain
-        if (!h->graphs.empty()) fail(RISA_ERR_UNSUPPORTED, "set the chain's variant before its first graph run");
+        int cur = 0;
+        ok(risa_hip_bp_get_variant(h->bp, &cur));
+        if (cur == variant) return;
+        h->drain();
+        h->release_graphs();
         ok(risa_hip_bp_set_variant(h->bp, variant));
+    });
+}
+
+/* Graph executables the chain holds (tooling; no reference counterpart). */
+extern "C" int risa_hip_chain_graph_count(risa_chain_t h, int* count)
+{
+    return guarded("risa_hip_chain_graph_count", [&] {
+        if (!h || !count) fail(RISA_ERR_ARG, "null argument");
+        *count = (int)h->graphs.size();
     });
 }
 

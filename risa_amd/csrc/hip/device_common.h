@@ -70,6 +70,18 @@ struct DeviceBuffer {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Waits for all work on `device` (the current device is restored).  Destroy
+// calls run this before releasing buffers, graphs or streams a launch on
+// any stream may still use: handles do not know their callers' streams.
+inline void drain_device(int device)
+{
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    if (prev != device) (void)hipSetDevice(device);
+    (void)hipDeviceSynchronize();
+    if (prev != device && prev >= 0) (void)hipSetDevice(prev);
+}
+
 inline void check_launch(const char* what)
 {
     hipError_t e = hipGetLastError();

@@ -8,7 +8,9 @@
 //
 // Output (j, i), j < P, receives half of internal sample A = (j, i) and half
 // of internal sample B = (j + P, D-1-i) -- the reference's two atomicAdd's
-// onto a zeroed buffer.  Each sample mixes up to two rays; each ray is a
+// onto a zeroed buffer.  (Odd D: the reference's mirror leaves output D/2
+// without B and gives output D/2 - 1 a second one, added by
+// rebin_extra_kernel.)  Each sample mixes up to two rays; each ray is a
 // bilinear interpolation of the fan sinogram in (theta, gamma).  The table
 // holds per output four ray slots of {packed indices, f_theta, f_gamma}:
 // 48 B/output instead of the reference's 2 x 56 B.
@@ -173,6 +175,34 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
     }
 }
 
+// Odd D: internal sample (j + P, D - 1) is the second addend of output
+// (j, D/2 - 1) beside the main table's (the reference's mirror sends two
+// second-half samples there, cuda_kernels_fan2para.h:164-176).  One thread per
+// row j adds it after rebin_kernel has written the row, in the reference's
+// order (first-half sample, then i = D/2, then i = D - 1).
+__global__ void __launch_bounds__(256) rebin_extra_kernel(const float* __restrict__ fan,
+                                                          const uint4* __restrict__ extra, float* __restrict__ par,
+                                                          int Df, int Pf, int D, int P, FrameGroups fg)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.y;
+    if (j >= P) return;
+    const int plane = fg.plane[grp];
+    const uint4* q = extra + ((long)plane * P + j) * 3;
+    const uint4 idx = q[0], ftb = q[1], fgb = q[2];
+    const bool r1 = (idx.x >> 31) != 0, r2 = (idx.y >> 31) != 0;
+    if (!r1 && !r2) return;   // adds +0.0f: the output keeps its bits
+    const long e = (long)j * D + (D / 2 - 1);
+    for (int u = 0; u < fg.count[grp]; u++) {
+        const int f = fg.frame[fg.start[grp] + u];
+        const float* fr = fan + (long)f * Df * Pf;
+        const float z1 = r1 ? rebin_ray(fr, idx.x, __uint_as_float(ftb.x), __uint_as_float(fgb.x), Df, Pf) : 0.0f;
+        const float z2 = r2 ? rebin_ray(fr, idx.y, __uint_as_float(ftb.y), __uint_as_float(fgb.y), Df, Pf) : 0.0f;
+        float* out = par + (long)f * D * P + e;
+        *out = *out + 0.5f * mix_rays(r1, r2, z1, z2);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Interpolation weights in FP64 (built once per geometry).
 //
@@ -204,18 +234,32 @@ __device__ double ring_angle64(double alpha, double DX, double DZ, double diam)
     return alpha <= M_PI ? acos(ae) : 2.0 * M_PI - acos(ae);
 }
 
-__global__ void rebin_weights_f64_kernel(uint32_t* __restrict__ q, RebinGeo g)
+// Second-half source detector of output t in the fold's permutation part
+// (the host's fold_source; -1 for none).
+__device__ __forceinline__ int fold_source_dev(int t, int D)
+{
+    const int h = D / 2;
+    if (D % 2 == 0) return D - 1 - t;
+    if (t == h) return -1;
+    return t > h ? 2 * h - t : 2 * h - 1 - t;
+}
+
+// extra = 0: the main table [P][D] (slots 0/1 internal sample (j, i), 2/3
+// (j + P, fold_source(i))); extra = 1: the odd-D table [P] (slots 0/1 internal
+// sample (j + P, D - 1)).
+__global__ void rebin_weights_f64_kernel(uint32_t* __restrict__ q, RebinGeo g, int extra)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)g.P * g.D * 4) return;
+    const long entries = extra ? (long)g.P : (long)g.P * g.D;
+    if (e >= entries * 4) return;
     const long out = e / 4;
     const int slot = (int)(e % 4);
-    const int j = (int)(out / g.D), i = (int)(out % g.D);
+    const int j = extra ? (int)out : (int)(out / g.D), i = extra ? g.D - 1 : (int)(out % g.D);
     uint32_t* w = q + out * 12;
     const uint32_t idx = w[slot];
     if (!(idx >> 31)) return;
-    const int jj = slot < 2 ? j : j + g.P;
-    const int ii = slot < 2 ? i : g.D - 1 - i;
+    const int jj = (slot < 2 && !extra) ? j : j + g.P;
+    const int ii = (slot < 2 || extra) ? i : fold_source_dev(i, g.D);
     const bool ray1 = (slot % 2) == 0;
     const double twoPi = 2.0 * M_PI;
     double alpha = 0.0;   // alpha[0] stays 0 in the reference table
@@ -268,6 +312,7 @@ struct risa_fan2para_s {
     risa::DeviceBuffer<int4> boxes;       // [planes][block rows][block cols][4 slots]
     long box_plane = 0;
     risa::DeviceBuffer<int2> order;       // [planes][blocks]: (block col, block row), global-gather blocks first
+    risa::DeviceBuffer<uint32_t> extra;   // odd D: [planes][P][12], the second addend of output (j, D/2 - 1)
 };
 
 using namespace risa;
@@ -329,6 +374,13 @@ extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int w
                 const PackedRebin pk = pack_rebin(*g, t, weights == RISA_REBIN_WEIGHTS_FP64);
                 uint32_t* dst = h->table.ptr + k * h->plane_words;
                 RISA_HIP_CHECK(hipMemcpy(dst, pk.q.data(), pk.q.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+                uint32_t* xdst = nullptr;
+                if (!pk.extra.empty()) {
+                    if (k == 0) h->extra.resize(pk.extra.size() * g->numberOfPlanes);
+                    xdst = h->extra.ptr + k * pk.extra.size();
+                    RISA_HIP_CHECK(hipMemcpy(xdst, pk.extra.data(), pk.extra.size() * sizeof(uint32_t),
+                                             hipMemcpyHostToDevice));
+                }
                 const std::vector<int4> bx = patch_boxes(pk, P, D, g->numberOfFanProjections, g->numberOfFanDetectors);
                 if (k == 0) {
                     h->box_plane = (long)bx.size();
@@ -365,8 +417,14 @@ extern "C" int risa_hip_fan2para_create_ex(const risa_geom* g, int device, int w
                     rg.kappa = cy != 0 ? std::atan(cx / cy) + (cy < 0 ? M_PI : 0.0) : (cx < 0 ? -M_PI / 2 : M_PI / 2);
                     if (cx == 0 && cy == 0) rg.kappa = 0.0;
                     const long n = (long)P * D * 4;
-                    hipLaunchKernelGGL(rebin_weights_f64_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, nullptr, dst, rg);
+                    hipLaunchKernelGGL(rebin_weights_f64_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, nullptr, dst, rg,
+                                       0);
                     check_launch("rebin_weights_f64_kernel");
+                    if (xdst) {
+                        hipLaunchKernelGGL(rebin_weights_f64_kernel, dim3(ceil_div((long)P * 4, 256)), dim3(256), 0,
+                                           nullptr, xdst, rg, 1);
+                        check_launch("rebin_weights_f64_kernel");
+                    }
                     RISA_HIP_CHECK(hipDeviceSynchronize());
                 }
             }
@@ -412,6 +470,12 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
                            reinterpret_cast<const uint4*>(h->table.ptr), h->boxes.ptr, h->order.ptr,
                            par + (size_t)f0 * D * P, Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
         check_launch("rebin_kernel");
+        if (D % 2 != 0) {
+            hipLaunchKernelGGL(rebin_extra_kernel, dim3(ceil_div(P, 256), groups), dim3(256), 0, s,
+                               fan + (size_t)f0 * Df * Pf, reinterpret_cast<const uint4*>(h->extra.ptr),
+                               par + (size_t)f0 * D * P, Df, Pf, D, P, fg);
+            check_launch("rebin_extra_kernel");
+        }
     }
 }
 
@@ -443,5 +507,8 @@ extern "C" int risa_hip_fan2para_table_bytes(risa_fan2para_t h, size_t* bytes)
 
 extern "C" int risa_hip_fan2para_destroy(risa_fan2para_t h)
 {
-    return guarded("risa_hip_fan2para_destroy", [&] { delete h; });
+    return guarded("risa_hip_fan2para_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }

@@ -124,43 +124,50 @@ __device__ __forceinline__ void stockham_pass(const float2* src, float2* dst, in
 }
 
 // rows [rows][D] in -> out (FP32 and/or FP16*out_scale); rows are paired.
+// The transform length M = plan.D is D itself, or (for lengths the radices
+// do not factor) an M >= 2D - 1 over which the length-D circular convolution
+// is exact: the row is zero-padded to M, H is the response of the wrapped
+// circulant on M points, and the first D outputs are kept.  A workgroup of
+// blockDim.x / 128 slots holds one row pair per slot.
 __global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict__ in, float* __restrict__ out,
                                                          uint16_t* __restrict__ out16, float out_scale,
                                                          const float2* __restrict__ tw, const float* __restrict__ H,
-                                                         FftPlan plan, long rows)
+                                                         FftPlan plan, int D, long rows)
 {
     extern __shared__ float2 lds[];
-    const int D = plan.D;
+    const int M = plan.D;
     const int T = 128;
-    const int slot = threadIdx.x / T;         // two row pairs per workgroup
+    const int slots = blockDim.x / T;
+    const int slot = threadIdx.x / T;
     const int t0 = threadIdx.x % T;
-    const long pair = (long)blockIdx.x * 2 + slot;
+    const long pair = (long)blockIdx.x * slots + slot;
     const long ra = pair * 2, rb = pair * 2 + 1;
-    float2* b0 = lds + (size_t)slot * 2 * D;
-    float2* b1 = b0 + D;
+    float2* b0 = lds + (size_t)slot * 2 * M;
+    float2* b1 = b0 + M;
     const bool active = ra < rows;
     const bool hasb = rb < rows;
     if (active) {
         const float* xa = in + ra * D;
         const float* xb = in + rb * D;
         for (int n = t0; n < D; n += T) b0[n] = make_float2(xa[n], hasb ? xb[n] : 0.0f);
+        for (int n = D + t0; n < M; n += T) b0[n] = make_float2(0.0f, 0.0f);
     }
     __syncthreads();
     float2* src = b0;
     float2* dst = b1;
     int Ns = 1;
     for (int s = 0; s < plan.nst; s++) {
-        if (active) stockham_pass<-1>(src, dst, D, plan.radix[s], Ns, tw, t0, T);
+        if (active) stockham_pass<-1>(src, dst, M, plan.radix[s], Ns, tw, t0, T);
         Ns *= plan.radix[s];
         float2* t = src; src = dst; dst = t;
         __syncthreads();
     }
     if (active)
-        for (int k = t0; k < D; k += T) src[k] = cscale(src[k], H[k]);
+        for (int k = t0; k < M; k += T) src[k] = cscale(src[k], H[k]);
     __syncthreads();
     Ns = 1;
     for (int s = 0; s < plan.nst; s++) {
-        if (active) stockham_pass<+1>(src, dst, D, plan.radix[s], Ns, tw, t0, T);
+        if (active) stockham_pass<+1>(src, dst, M, plan.radix[s], Ns, tw, t0, T);
         Ns *= plan.radix[s];
         float2* t = src; src = dst; dst = t;
         __syncthreads();
@@ -263,7 +270,12 @@ __global__ void __launch_bounds__(64) fft512_filter_kernel(const float* __restri
 // Gradient w.r.t. the taps: for each row slot the complex row is
 // z = x + i sigma g (one row); X[k] = (Z[k] + conj Z[D-k]) / 2,
 // G[k] = (Z[k] - conj Z[D-k]) / 2i, contribution Re(conj(G[k]) X[k]).
-// Per-workgroup partial sums -> partial[blockIdx.x][k].
+// A fixed grid walks the rows (workgroup b takes rows b*slots + slot,
+// advancing by grid*slots, in order) and keeps its sums in registers:
+// partial[blockIdx.x][k], so the partial buffer is sized once, not per call.
+// D = plan.D <= 2048.
+constexpr int kTapBlocks = 512;
+constexpr int kTapPer = 9;   // taps per thread: ceil((2048 / 2 + 1) / 128)
 __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restrict__ x, const float* __restrict__ g,
                                                             const float2* __restrict__ tw, FftPlan plan, long rows,
                                                             float* __restrict__ partial)
@@ -271,63 +283,89 @@ __global__ void __launch_bounds__(256) fft_grad_taps_kernel(const float* __restr
     extern __shared__ float2 lds[];
     const int D = plan.D, K = D / 2 + 1;
     const int T = 128;
+    const int slots = blockDim.x / T;
     const int slot = threadIdx.x / T;
     const int t0 = threadIdx.x % T;
-    const long r = (long)blockIdx.x * 2 + slot;
     float2* b0 = lds + (size_t)slot * 2 * D;
     float2* b1 = b0 + D;
-    const bool active = r < rows;
-    // x and g can differ in magnitude by many orders (g is a loss gradient):
-    // g is scaled by a power of two sigma that brings its row maximum to x's,
-    // or the rounding of X would swamp G in the shared complex transform
     __shared__ unsigned int amax[2][2];
-    if (threadIdx.x < 4) amax[threadIdx.x / 2][threadIdx.x % 2] = 0u;
-    __syncthreads();
-    float mx = 0.0f, mg = 0.0f;
-    if (active)
-        for (int n = t0; n < D; n += T) {
-            const float xv = x[r * D + n], gv = g[r * D + n];
-            b0[n] = make_float2(xv, gv);
-            mx = fmaxf(mx, fabsf(xv));
-            mg = fmaxf(mg, fabsf(gv));
+    float acc[kTapPer];
+#pragma unroll
+    for (int u = 0; u < kTapPer; u++) acc[u] = 0.0f;
+    for (long r0 = (long)blockIdx.x * slots; r0 < rows; r0 += (long)gridDim.x * slots) {
+        const long r = r0 + slot;
+        const bool active = r < rows;
+        // x and g can differ in magnitude by many orders (g is a loss
+        // gradient): g is scaled by a power of two sigma that brings its row
+        // maximum to x's, or the rounding of X would swamp G in the shared
+        // complex transform
+        if (threadIdx.x < 4) amax[threadIdx.x / 2][threadIdx.x % 2] = 0u;
+        __syncthreads();
+        float mx = 0.0f, mg = 0.0f;
+        if (active)
+            for (int n = t0; n < D; n += T) {
+                const float xv = x[r * D + n], gv = g[r * D + n];
+                b0[n] = make_float2(xv, gv);
+                mx = fmaxf(mx, fabsf(xv));
+                mg = fmaxf(mg, fabsf(gv));
+            }
+        atomicMax(&amax[slot][0], __float_as_uint(mx));
+        atomicMax(&amax[slot][1], __float_as_uint(mg));
+        __syncthreads();
+        int e = 0;
+        {
+            const float ax = __uint_as_float(amax[slot][0]), ag = __uint_as_float(amax[slot][1]);
+            if (ax > 0.0f && ag > 0.0f) e = max(-100, min(100, ilogbf(ax) - ilogbf(ag)));
         }
-    atomicMax(&amax[slot][0], __float_as_uint(mx));
-    atomicMax(&amax[slot][1], __float_as_uint(mg));
-    __syncthreads();
-    int e = 0;
-    {
-        const float ax = __uint_as_float(amax[slot][0]), ag = __uint_as_float(amax[slot][1]);
-        if (ax > 0.0f && ag > 0.0f) e = max(-100, min(100, ilogbf(ax) - ilogbf(ag)));
+        if (active && e != 0)
+            for (int n = t0; n < D; n += T) b0[n].y = ldexpf(b0[n].y, e);
+        __syncthreads();
+        float2* src = b0;
+        float2* dst = b1;
+        int Ns = 1;
+        for (int s = 0; s < plan.nst; s++) {
+            if (active) stockham_pass<-1>(src, dst, D, plan.radix[s], Ns, tw, t0, T);
+            Ns *= plan.radix[s];
+            float2* t = src; src = dst; dst = t;
+            __syncthreads();
+        }
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < kTapPer; u++) {
+                const int k = t0 + u * T;
+                if (k >= K) break;
+                const float2 zk = src[k];
+                const float2 zm = src[(D - k) % D];
+                const float2 X = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+                const float2 G = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+                acc[u] += ldexpf(G.x * X.x + G.y * X.y, -e);
+            }
+        }
+        __syncthreads();   // the next pair overwrites b0 / b1
     }
-    if (active && e != 0)
-        for (int n = t0; n < D; n += T) b0[n].y = ldexpf(b0[n].y, e);
-    __syncthreads();
-    float2* src = b0;
-    float2* dst = b1;
-    int Ns = 1;
-    for (int s = 0; s < plan.nst; s++) {
-        if (active) stockham_pass<-1>(src, dst, D, plan.radix[s], Ns, tw, t0, T);
-        Ns *= plan.radix[s];
-        float2* t = src; src = dst; dst = t;
+    // two slots' sums through LDS (slot 1 writes, slot 0 adds)
+    float* red = reinterpret_cast<float*>(lds);
+    if (slots == 2) {
+        if (slot == 1)
+#pragma unroll
+            for (int u = 0; u < kTapPer; u++) {
+                const int k = t0 + u * T;
+                if (k < K) red[k] = acc[u];
+            }
         __syncthreads();
     }
-    // per-slot contributions into dst (scratch), then the two slots summed
-    for (int k = t0; k < K; k += T) {
-        float c = 0.0f;
-        if (active) {
-            const float2 zk = src[k];
-            const float2 zm = src[(D - k) % D];
-            const float2 X = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            const float2 G = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-            c = ldexpf(G.x * X.x + G.y * X.y, -e);
-        }
-        dst[k].x = c;
-    }
-    __syncthreads();
-    // both slots ran the same passes, so slot 1's scratch sits 2*D further on
     if (slot == 0)
-        for (int k = t0; k < K; k += T) partial[(long)blockIdx.x * K + k] = dst[k].x + dst[k + 2 * D].x;
+#pragma unroll
+        for (int u = 0; u < kTapPer; u++) {
+            const int k = t0 + u * T;
+            if (k < K) partial[(long)blockIdx.x * K + k] = slots == 2 ? acc[u] + red[k] : acc[u];
+        }
 }
+
+// Row slots per workgroup of the FFT kernels: two while their buffers (two
+// per slot, M complex entries each) and a little static LDS fit 64 KB, else
+// one (gfx950 gives a workgroup up to 160 KB).
+inline int fft_slots(int M) { return (size_t)2 * 2 * M * sizeof(float2) + 64 <= 64 * 1024 ? 2 : 1; }
 
 // One workgroup per tap k: thread t sums partials t, t + 256, ... in order,
 // then a fixed LDS tree -- deterministic, and no thread walks all partials.
@@ -348,6 +386,151 @@ __global__ void __launch_bounds__(256) reduce_taps_kernel(const float* __restric
         const float wk = (k == 0 || (D % 2 == 0 && k == D / 2)) ? 1.0f : 2.0f;
         dtaps[k] = wk * sum[0];
     }
+}
+
+// Tap gradient for a padded transform length M != D.  dL/dh_k =
+// w_k sum_d R[d] cos(2 pi k d / D) (w_0 = 1, w_{D/2} = 1 for even D, else 2),
+// where R[d] = sum_rows sum_n g[n] x[(n - d) mod D] is the circular
+// cross-correlation summed over rows.  Per row, with z = x + i sigma g
+// zero-padded to M >= 2D - 1, the linear cross-correlation r_lin[j]
+// (|j| < D) has the spectrum G(k) conj(X(k)); the spectra are summed over rows
+// (fixed grid, registers, then a fixed-order reduction), transformed back
+// once, wrapped to R[d] = r_lin[d] + r_lin[d - D], and cosine-transformed.
+constexpr int kXspecPer = 17;   // bins per thread: ceil((4096 / 2 + 1) / 128)
+__global__ void __launch_bounds__(256) fft_xspec_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                                        const float2* __restrict__ tw, FftPlan plan, int D, long rows,
+                                                        float2* __restrict__ partial)
+{
+    extern __shared__ float2 lds[];
+    const int M = plan.D, K = M / 2 + 1;
+    const int T = 128;
+    const int slots = blockDim.x / T;
+    const int slot = threadIdx.x / T;
+    const int t0 = threadIdx.x % T;
+    float2* b0 = lds + (size_t)slot * 2 * M;
+    float2* b1 = b0 + M;
+    __shared__ unsigned int amax[2][2];
+    float2 acc[kXspecPer];
+#pragma unroll
+    for (int u = 0; u < kXspecPer; u++) acc[u] = make_float2(0.0f, 0.0f);
+    for (long r0 = (long)blockIdx.x * slots; r0 < rows; r0 += (long)gridDim.x * slots) {
+        const long r = r0 + slot;
+        const bool active = r < rows;
+        if (threadIdx.x < 4) amax[threadIdx.x / 2][threadIdx.x % 2] = 0u;
+        __syncthreads();
+        float mx = 0.0f, mg = 0.0f;
+        if (active) {
+            for (int n = t0; n < D; n += T) {
+                const float xv = x[r * D + n], gv = g[r * D + n];
+                b0[n] = make_float2(xv, gv);
+                mx = fmaxf(mx, fabsf(xv));
+                mg = fmaxf(mg, fabsf(gv));
+            }
+            for (int n = D + t0; n < M; n += T) b0[n] = make_float2(0.0f, 0.0f);
+        }
+        atomicMax(&amax[slot][0], __float_as_uint(mx));
+        atomicMax(&amax[slot][1], __float_as_uint(mg));
+        __syncthreads();
+        int e = 0;
+        {
+            const float ax = __uint_as_float(amax[slot][0]), ag = __uint_as_float(amax[slot][1]);
+            if (ax > 0.0f && ag > 0.0f) e = max(-100, min(100, ilogbf(ax) - ilogbf(ag)));
+        }
+        if (active && e != 0)
+            for (int n = t0; n < D; n += T) b0[n].y = ldexpf(b0[n].y, e);
+        __syncthreads();
+        float2* src = b0;
+        float2* dst = b1;
+        int Ns = 1;
+        for (int s = 0; s < plan.nst; s++) {
+            if (active) stockham_pass<-1>(src, dst, M, plan.radix[s], Ns, tw, t0, T);
+            Ns *= plan.radix[s];
+            float2* t = src; src = dst; dst = t;
+            __syncthreads();
+        }
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < kXspecPer; u++) {
+                const int k = t0 + u * T;
+                if (k >= K) break;
+                const float2 zk = src[k];
+                const float2 zm = src[(M - k) % M];
+                const float2 X = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+                const float2 G = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+                // G conj(X), G carrying sigma = 2^e
+                acc[u].x += ldexpf(G.x * X.x + G.y * X.y, -e);
+                acc[u].y += ldexpf(G.y * X.x - G.x * X.y, -e);
+            }
+        }
+        __syncthreads();
+    }
+    float2* red = lds;
+    if (slots == 2) {
+        if (slot == 1)
+#pragma unroll
+            for (int u = 0; u < kXspecPer; u++) {
+                const int k = t0 + u * T;
+                if (k < K) red[k] = acc[u];
+            }
+        __syncthreads();
+    }
+    if (slot == 0)
+#pragma unroll
+        for (int u = 0; u < kXspecPer; u++) {
+            const int k = t0 + u * T;
+            if (k >= K) break;
+            float2 v = acc[u];
+            if (slots == 2) v = make_float2(v.x + red[k].x, v.y + red[k].y);
+            partial[(long)blockIdx.x * K + k] = v;
+        }
+}
+
+// Bin k of the summed spectrum: the partials in block order, in double.
+__global__ void xspec_reduce_kernel(const float2* __restrict__ partial, int nblocks, int K, double2* __restrict__ sum)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    double sx = 0.0, sy = 0.0;
+    for (int b = 0; b < nblocks; b++) {
+        const float2 v = partial[(long)b * K + k];
+        sx += v.x;
+        sy += v.y;
+    }
+    sum[k] = make_double2(sx, sy);
+}
+
+// R[d] = r_lin[d] + r_lin[d - D], r_lin[j] = (1/M) sum_k S(k) e^{2 pi i j k / M}
+// over the Hermitian spectrum S (S(M - k) = conj S(k)).
+__global__ void xcorr_wrap_kernel(const double2* __restrict__ S, int M, int D, double* __restrict__ R)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    double acc = 0.0;
+    for (int w = 0; w < 2; w++) {
+        const long j = w == 0 ? d : (long)d - D + M;   // r_lin[d - D] sits at index d - D + M
+        double v = S[0].x;
+        const int kmax = (M % 2 == 0) ? M / 2 - 1 : M / 2;
+        for (int k = 1; k <= kmax; k++) {
+            double sn, cs;
+            sincospi(2.0 * (double)((j * k) % M) / M, &sn, &cs);
+            v += 2.0 * (S[k].x * cs - S[k].y * sn);
+        }
+        if (M % 2 == 0) v += S[M / 2].x * ((j % 2) ? -1.0 : 1.0);
+        acc += v;
+    }
+    R[d] = acc / M;
+}
+
+// dtaps[k] = w_k sum_d R[d] cos(2 pi k d / D).
+__global__ void xcorr_taps_kernel(const double* __restrict__ R, int D, float* __restrict__ dtaps)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = D / 2 + 1;
+    if (k >= K) return;
+    double acc = 0.0;
+    for (int d = 0; d < D; d++) acc += R[d] * cospi(2.0 * (double)(((long)k * d) % D) / D);
+    const double wk = (k == 0 || (D % 2 == 0 && k == D / 2)) ? 1.0 : 2.0;
+    dtaps[k] = (float)(wk * acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -686,14 +869,18 @@ struct risa_filter_s {
     int mode = 0;
     risa_geom g{};
     std::vector<float> taps;
+    int M = 0;                            // FFT length: D, or a padded M >= 2D - 1
     risa::FftPlan plan{};
-    risa::DeviceBuffer<float2> tw;        // exp(-2 pi i n / D)
-    risa::DeviceBuffer<float> H;          // full symmetric response [D]
+    risa::DeviceBuffer<float2> tw;        // exp(-2 pi i n / M)
+    risa::DeviceBuffer<float> H;          // full symmetric response [M]
     risa::DeviceBuffer<_Float16> Ch, Cl;    // split circulant as a D x D matrix (GEMM mode, other D)
     risa::DeviceBuffer<_Float16> ch1, cl1;  // split circulant vector c[d] (GEMM mode, 128 <= D <= 704)
     float sc = 1.0f;                      // circulant scale
-    risa::DeviceBuffer<float> scratch;    // grad partial sums
-    risa::DeviceBuffer<float> tmp;        // adjoint / out-of-place staging
+    risa::DeviceBuffer<float> scratch;    // grad partial sums [kTapBlocks][D/2+1]
+    risa::DeviceBuffer<float2> xpart;     // padded grad: cross spectra [kTapBlocks][M/2+1]
+    risa::DeviceBuffer<double2> xsum;     // padded grad: their sum [M/2+1]
+    risa::DeviceBuffer<double> xcorr;     // padded grad: wrapped cross-correlation [D]
+    risa::DeviceBuffer<float> stage;      // GEMM-mode in-place staging [kGemmStageRows][D]
 };
 
 using namespace risa;
@@ -710,24 +897,45 @@ static bool make_plan(int D, FftPlan& p)
     return n == 1;
 }
 
+// Circulant coefficients of the taps, c[d] = h0 + 2 sum_k h_k cos(2 pi k d / D)
+// + [D even] h_{D/2} (-1)^d, in double.
+static std::vector<double> circulant(const std::vector<float>& taps, int D)
+{
+    std::vector<double> c(D);
+    const int half = D / 2;
+    for (int d = 0; d < D; d++) {
+        double acc = taps[0];
+        const int kmax = (D % 2 == 0) ? half - 1 : half;
+        for (int k = 1; k <= kmax; k++) acc += 2.0 * taps[k] * std::cos(2.0 * M_PI * (double)(((long)k * d) % D) / D);
+        if (D % 2 == 0) acc += taps[half] * ((d % 2) ? -1.0 : 1.0);
+        c[d] = acc;
+    }
+    return c;
+}
+
 // Upload the response for the current taps (FFT table and, in GEMM mode, the
-// split circulant c[d] = h0 + 2 sum h_k cos(2 pi k d / D) + [D even] h_{D/2}(-1)^d).
+// split circulant c).
 static void upload_response(risa_filter_s* h)
 {
-    const int D = h->g.numberOfParallelDetectors;
-    std::vector<float> Hf(D);
-    for (int k = 0; k < D; k++) Hf[k] = h->taps[std::min(k, D - k)];
+    const int D = h->g.numberOfParallelDetectors, M = h->M;
+    std::vector<float> Hf(M);
+    std::vector<double> c;
+    if (M == D) {
+        for (int k = 0; k < D; k++) Hf[k] = h->taps[std::min(k, D - k)];
+    } else {
+        // padded: the response of c wrapped onto M points (c at lags
+        // -(D-1) .. D-1, zeros elsewhere), with the 1/M of the unnormalised
+        // inverse transform
+        c = circulant(h->taps, D);
+        for (int k = 0; k < M; k++) {
+            double acc = c[0];
+            for (int j = 1; j < D; j++) acc += 2.0 * c[j] * std::cos(2.0 * M_PI * (double)(((long)j * k) % M) / M);
+            Hf[k] = (float)(acc / M);
+        }
+    }
     h->H.upload(Hf);
     if (h->mode == RISA_FILTER_MODE_GEMM_SPLIT) {
-        std::vector<double> c(D);
-        const int half = D / 2;
-        for (int d = 0; d < D; d++) {
-            double acc = h->taps[0];
-            const int kmax = (D % 2 == 0) ? half - 1 : half;
-            for (int k = 1; k <= kmax; k++) acc += 2.0 * h->taps[k] * std::cos(2.0 * M_PI * (double)(((long)k * d) % D) / D);
-            if (D % 2 == 0) acc += h->taps[half] * ((d % 2) ? -1.0 : 1.0);
-            c[d] = acc;
-        }
+        if (c.empty()) c = circulant(h->taps, D);
         double cmax = 0.0;
         for (double v : c) cmax = std::max(cmax, std::fabs(v));
         // scale so the largest entry sits near 2^10 (well inside FP16 range)
@@ -755,6 +963,24 @@ static void upload_response(risa_filter_s* h)
     }
 }
 
+// Detector counts up to kMaxDetectors are filtered; the FFT a workgroup holds
+// in LDS (one row pair, two buffers) is at most kMaxFftLen long.
+constexpr int kMaxDetectors = 2048;
+constexpr int kMaxFftLen = 4096;
+// Rows per GEMM-mode in-place pass through the staging buffer.
+constexpr long kGemmStageRows = 16384;
+
+// FFT length of D: D itself when its radices are 2/3/5, else the smallest
+// 2/3/5-smooth M >= 2D - 1 (zero-padded exact circular convolution).
+static int fft_length(int D, FftPlan& plan)
+{
+    if (D < 1 || D > kMaxDetectors) return 0;
+    if (make_plan(D, plan)) return D;
+    for (int M = 2 * D - 1; M <= kMaxFftLen; M++)
+        if (make_plan(M, plan)) return M;
+    return 0;
+}
+
 extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, risa_filter_t* out)
 {
     return guarded("risa_hip_filter_create", [&] {
@@ -768,14 +994,17 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
             h->device = device;
             h->mode = mode;
             h->g = *g;
-            if (!make_plan(D, h->plan)) fail(RISA_ERR_UNSUPPORTED, "FFT length %d is not 2^a 3^b 5^c", D);
-            if ((size_t)D * 32 > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "FFT length %d exceeds the LDS budget", D);
-            std::vector<float2> tw(D);
-            for (int n = 0; n < D; n++)
-                tw[n] = make_float2((float)std::cos(-2.0 * M_PI * n / D), (float)std::sin(-2.0 * M_PI * n / D));
-            h->tw.resize(D);
+            // the FFT also serves the adjoint and the tap gradient in GEMM mode
+            h->M = fft_length(D, h->plan);
+            if (h->M == 0)
+                fail(RISA_ERR_UNSUPPORTED, "%d detectors: the filter takes 1 .. %d", D, kMaxDetectors);
+            const int M = h->M;
+            std::vector<float2> tw(M);
+            for (int n = 0; n < M; n++)
+                tw[n] = make_float2((float)std::cos(-2.0 * M_PI * n / M), (float)std::sin(-2.0 * M_PI * n / M));
+            h->tw.resize(M);
             h->tw.upload(tw);
-            h->H.resize(D);
+            h->H.resize(M);
             if (mode == RISA_FILTER_MODE_GEMM_SPLIT) {
                 if (D >= kCircMinD && D <= kCircMaxD) {
                     h->ch1.resize(D);
@@ -784,6 +1013,15 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
                     h->Ch.resize((size_t)D * D);
                     h->Cl.resize((size_t)D * D);
                 }
+                h->stage.resize((size_t)kGemmStageRows * D);
+            }
+            // tap-gradient sums, sized here so run calls allocate nothing
+            if (M == D) {
+                h->scratch.resize((size_t)kTapBlocks * (D / 2 + 1));
+            } else {
+                h->xpart.resize((size_t)kTapBlocks * (M / 2 + 1));
+                h->xsum.resize((size_t)(M / 2 + 1));
+                h->xcorr.resize((size_t)D);
             }
             h->taps = filter_taps(*g);
             upload_response(h);
@@ -816,18 +1054,19 @@ extern "C" int risa_hip_filter_get_taps(risa_filter_t h, float* taps)
 static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* out16, float out_scale, long rows,
                        hipStream_t s)
 {
-    const int D = h->g.numberOfParallelDetectors;
+    const int D = h->g.numberOfParallelDetectors, M = h->M;
     const long pairs = (rows + 1) / 2;
-    if (D == 512) {
+    if (D == 512 && M == 512) {
         hipLaunchKernelGGL(fft512_filter_kernel, dim3((unsigned)pairs), dim3(64), 0, s, in, out, out16, out_scale,
                            h->tw.ptr, h->H.ptr, rows);
         check_launch("fft512_filter_kernel");
         return;
     }
-    const int grid = ceil_div(pairs, 2);
-    const size_t lds = (size_t)2 * 2 * D * sizeof(float2);
-    hipLaunchKernelGGL(fft_filter_kernel, dim3(grid), dim3(256), lds, s, in, out, out16, out_scale, h->tw.ptr,
-                       h->H.ptr, h->plan, rows);
+    const int slots = fft_slots(M);
+    const int grid = ceil_div(pairs, slots);
+    const size_t lds = (size_t)slots * 2 * M * sizeof(float2);
+    hipLaunchKernelGGL(fft_filter_kernel, dim3(grid), dim3(128 * slots), lds, s, in, out, out16, out_scale, h->tw.ptr,
+                       h->H.ptr, h->plan, D, rows);
     check_launch("fft_filter_kernel");
 }
 
@@ -870,16 +1109,18 @@ extern "C" int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, voi
         if (h->mode == RISA_FILTER_MODE_FFT32) {
             launch_fft(h, sino, sino, nullptr, 1.0f, rows, as_stream(stream));
         } else {
-            // the GEMM reads whole rows of X per output tile: filter into a
-            // staging buffer (grown on first use), then copy back
-            const size_t n = (size_t)rows * h->g.numberOfParallelDetectors;
-            if (h->tmp.count < n) {
-                DeviceGuard dg(h->device);
-                RISA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
-                h->tmp.resize(n);
+            // the GEMM reads whole rows of X per output tile: each pass
+            // filters up to kGemmStageRows rows into the staging buffer
+            // (allocated at create) and copies them back; passes are
+            // stream-ordered, so the buffer is reused safely
+            const int D = h->g.numberOfParallelDetectors;
+            for (long r0 = 0; r0 < rows; r0 += kGemmStageRows) {
+                const long n = std::min(kGemmStageRows, rows - r0);
+                float* x = sino + r0 * D;
+                launch_gemm(h, x, gemm_out(h->stage.ptr, nullptr, 1.0f), n, as_stream(stream));
+                RISA_HIP_CHECK(hipMemcpyAsync(x, h->stage.ptr, (size_t)n * D * sizeof(float), hipMemcpyDeviceToDevice,
+                                              as_stream(stream)));
             }
-            launch_gemm(h, sino, gemm_out(h->tmp.ptr, nullptr, 1.0f), rows, as_stream(stream));
-            RISA_HIP_CHECK(hipMemcpyAsync(sino, h->tmp.ptr, n * sizeof(float), hipMemcpyDeviceToDevice, as_stream(stream)));
         }
     });
 }
@@ -932,30 +1173,52 @@ extern "C" int risa_hip_filter_adjoint_run(risa_filter_t h, float* grad, int sli
     });
 }
 
+static void launch_grad_taps_padded(risa_filter_t h, const float* x, const float* g, long rows, int nblocks,
+                                    float* dtaps, hipStream_t s)
+{
+    const int D = h->g.numberOfParallelDetectors, M = h->M, KM = M / 2 + 1;
+    const int slots = fft_slots(M);
+    const size_t lds = (size_t)slots * 2 * M * sizeof(float2);
+    const int grid = (int)std::max(1L, std::min<long>(nblocks, (rows + slots - 1) / slots));
+    hipLaunchKernelGGL(fft_xspec_kernel, dim3(grid), dim3(128 * slots), lds, s, x, g, h->tw.ptr, h->plan, D, rows,
+                       h->xpart.ptr);
+    check_launch("fft_xspec_kernel");
+    hipLaunchKernelGGL(xspec_reduce_kernel, dim3(ceil_div(KM, 256)), dim3(256), 0, s, h->xpart.ptr, grid, KM,
+                       h->xsum.ptr);
+    check_launch("xspec_reduce_kernel");
+    hipLaunchKernelGGL(xcorr_wrap_kernel, dim3(ceil_div(D, 64)), dim3(64), 0, s, h->xsum.ptr, M, D, h->xcorr.ptr);
+    check_launch("xcorr_wrap_kernel");
+    hipLaunchKernelGGL(xcorr_taps_kernel, dim3(ceil_div(D / 2 + 1, 64)), dim3(64), 0, s, h->xcorr.ptr, D, dtaps);
+    check_launch("xcorr_taps_kernel");
+}
+
 extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const float* g, int slices, float* dtaps,
                                          void* stream)
 {
     return guarded("risa_hip_filter_grad_taps", [&] {
         if (!h || !x || !g || !dtaps || slices < 0) fail(RISA_ERR_ARG, "bad argument");
-        DeviceGuard dg(h->device);
-        const int D = h->g.numberOfParallelDetectors, K = D / 2 + 1;
+        const int D = h->g.numberOfParallelDetectors, K = D / 2 + 1, M = h->M;
         const long rows = (long)slices * h->g.numberOfParallelProjections;
-        const int nblocks = std::max(1, ceil_div(rows, 2));
-        if (h->scratch.count < (size_t)nblocks * K) {
-            RISA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
-            h->scratch.resize((size_t)nblocks * K);
+        hipStream_t s = as_stream(stream);
+        if (M != D) {
+            launch_grad_taps_padded(h, x, g, rows, kTapBlocks, dtaps, s);
+            return;
         }
-        const size_t lds = (size_t)2 * 2 * D * sizeof(float2);
-        hipLaunchKernelGGL(fft_grad_taps_kernel, dim3(nblocks), dim3(256), lds, as_stream(stream), x, g, h->tw.ptr,
-                           h->plan, rows, h->scratch.ptr);
+        const int slots = fft_slots(D);
+        const int nblocks = (int)std::max(1L, std::min<long>(kTapBlocks, (rows + slots - 1) / slots));
+        const size_t lds = (size_t)slots * 2 * D * sizeof(float2);
+        hipLaunchKernelGGL(fft_grad_taps_kernel, dim3(nblocks), dim3(128 * slots), lds, s, x, g, h->tw.ptr, h->plan,
+                           rows, h->scratch.ptr);
         check_launch("fft_grad_taps_kernel");
-        hipLaunchKernelGGL(reduce_taps_kernel, dim3(K), dim3(256), 0, as_stream(stream), h->scratch.ptr, nblocks, K, D,
-                           dtaps);
+        hipLaunchKernelGGL(reduce_taps_kernel, dim3(K), dim3(256), 0, s, h->scratch.ptr, nblocks, K, D, dtaps);
         check_launch("reduce_taps_kernel");
     });
 }
 
 extern "C" int risa_hip_filter_destroy(risa_filter_t h)
 {
-    return guarded("risa_hip_filter_destroy", [&] { delete h; });
+    return guarded("risa_hip_filter_destroy", [&] {
+        if (h) drain_device(h->device);
+        delete h;
+    });
 }

@@ -330,7 +330,7 @@ RebinTables rebin_tables(const risa_geom& g, int k)
 PackedRebin pack_rebin(const risa_geom& g, const RebinTables& t, bool goal_angles)
 {
     const int P = g.numberOfParallelProjections, D = t.D, Pf = t.Pf, Df = t.Df;
-    if (D % 2 != 0) fail(RISA_ERR_UNSUPPORTED, "Fan2Para needs an even numberOfParallelDetectors (got %d)", D);
+    if (D < 2) fail(RISA_ERR_UNSUPPORTED, "Fan2Para needs at least 2 parallel detectors (got %d)", D);
     if (Pf > 32767 || Df > 32767) fail(RISA_ERR_UNSUPPORTED, "fan sinogram larger than 32767 in one dimension");
     PackedRebin pk;
     pk.q.assign((size_t)P * D * 12, 0u);
@@ -360,14 +360,29 @@ PackedRebin pack_rebin(const risa_geom& g, const RebinTables& t, bool goal_angle
         for (int i = 0; i < D; i++) {
             uint32_t* q = &pk.q[((size_t)j * D + i) * 12];
             const size_t eA = (size_t)j * D + i;                  // first half, same detector
-            const size_t eB = (size_t)(j + P) * D + (D - 1 - i);  // second half, mirrored (:160-176)
-            float f[8];
+            const int src = fold_source(i, D);                    // second half, mirrored (:160-176)
+            float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             encode(eA, 1, q[0], f[0], f[4]);
             encode(eA, 2, q[1], f[1], f[5]);
-            encode(eB, 1, q[2], f[2], f[6]);
-            encode(eB, 2, q[3], f[3], f[7]);
+            if (src >= 0) {
+                const size_t eB = (size_t)(j + P) * D + src;
+                encode(eB, 1, q[2], f[2], f[6]);
+                encode(eB, 2, q[3], f[3], f[7]);
+            }
             std::memcpy(&q[4], f, sizeof(f));
         }
+    if (D % 2 != 0) {
+        // odd D: internal sample (j + P, D - 1) also folds onto output (j, D/2 - 1)
+        pk.extra.assign((size_t)P * 12, 0u);
+        for (int j = 0; j < P; j++) {
+            uint32_t* q = &pk.extra[(size_t)j * 12];
+            const size_t e = (size_t)(j + P) * D + (D - 1);
+            float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            encode(e, 1, q[0], f[0], f[4]);
+            encode(e, 2, q[1], f[1], f[5]);
+            std::memcpy(&q[4], f, sizeof(f));
+        }
+    }
     return pk;
 }
 

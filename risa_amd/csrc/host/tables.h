@@ -28,12 +28,31 @@ struct RebinTables {
 };
 RebinTables rebin_tables(const risa_geom& g, int plane);
 
+// The 360 -> 180 degree fold of the reference (cuda_kernels_fan2para.h:160-176):
+// internal sample (j + P, i) is added to output (j, mirror(i)) with
+// mirror(i) = D - 1 - i for i < D/2, else D/2 - 1 - i % (D/2).  For even D
+// this is D - 1 - i, a permutation; for odd D = 2h + 1 output h receives no
+// second-half sample and output h - 1 receives two (from i = h and i = 2h).
+// fold_source(t, D) is the source of output t in the permutation part
+// (-1 for none); the odd-D extra source of output h - 1 is D - 1.
+inline int fold_source(int t, int D)
+{
+    const int h = D / 2;
+    if (D % 2 == 0) return D - 1 - t;
+    if (t == h) return -1;
+    return t > h ? 2 * h - t : 2 * h - 1 - t;
+}
+
 // Packed gather table of one plane, one entry per output (j < P, i < D):
 //   q0 = {idx A1, idx A2, idx B1, idx B2}, q1 = {f_theta x4}, q2 = {f_gamma x4}
-// where A = internal sample (j, i), B = internal sample (j + P, D-1-i), and
-// idx = theta_before | gamma_before << 16 | valid << 31 (0 = ray unused).
+// where A = internal sample (j, i), B = internal sample
+// (j + P, fold_source(i)), and idx = theta_before | gamma_before << 16 |
+// valid << 31 (0 = ray unused).  For odd D, `extra` holds per row j one more
+// entry of the same form whose slots 0/1 are the rays of internal sample
+// (j + P, D - 1), added to output (j, D/2 - 1).
 struct PackedRebin {
-    std::vector<uint32_t> q;   // [P][D][12] 32-bit words
+    std::vector<uint32_t> q;       // [P][D][12] 32-bit words
+    std::vector<uint32_t> extra;   // odd D: [P][12]; empty for even D
 };
 // goal_angles = true leaves each ray's float goal angles (theta, gamma) in
 // the weight slots, for the FP64 weight pass on the device to replace.

@@ -1,6 +1,7 @@
 // Stage implementations over the C-ABI (see risa/Stages.h).
 #include "risa/Stages.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
 #include <stdexcept>
@@ -144,7 +145,19 @@ H2D::H2D(const std::string& configFile)
                   "hipMemcpyAsync");
     });
 }
-H2D::~H2D() { stop(); }
+H2D::~H2D()
+{
+    stop();
+    std::fprintf(stderr, "H2D: WorstCaseTime: %gs; BestCaseTime: %gs;\n", loss_.worst_case_seconds(),
+                 loss_.best_case_seconds());
+    std::fprintf(stderr, "H2D: Could not reconstruct %lld elements; %g%% loss\n", loss_.lost(), loss_.loss_percent());
+}
+
+void H2D::process(input_type&& img)
+{
+    if (img.valid()) loss_.arrived(img.index());
+    DeviceStage::process(std::move(img));
+}
 
 // ----------------------------------------------------------- Reordering --
 Reordering::Reordering(const std::string& configFile)
@@ -289,6 +302,66 @@ Filter::~Filter()
     for (auto h : h_) risa_hip_filter_destroy(h);
 }
 
+// ----------------------------------------------------- mixed precision --
+bool mixed_precision(const std::string& configFile)
+{
+    const std::string p = config_string_or(configFile, "precision", "fp32");
+    if (p == "fp32") return false;
+    if (p == "mixed") return true;
+    throw std::runtime_error("precision must be \"fp32\" or \"mixed\", not \"" + p + "\"");
+}
+
+FilterMixed::FilterMixed(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int pool = config_int_or(configFile, "memPoolSize_filter", 16);
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
+        risa_filter_t h = nullptr;
+        check(risa_hip_filter_create(&g_, w / lanes, RISA_FILTER_MODE_GEMM_SPLIT, &h), "FilterMixed");
+        h_.push_back(h);
+    }
+    start("FilterMixed", devices, lanes, pool, par_elems(g_),
+          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_filter_run_f16(h_[(std::size_t)w], in.data(), out.data(), 1, kMixedSinoScale, s),
+                    "FilterMixed::processor");
+          });
+}
+
+FilterMixed::~FilterMixed()
+{
+    stop();
+    for (auto h : h_) risa_hip_filter_destroy(h);
+}
+
+BackprojectionMixed::BackprojectionMixed(const std::string& configFile)
+{
+    g_ = read_geometry(configFile);
+    const int pool = config_int(configFile, "memPoolSize_backProjection");
+    const int devices = device_count(), lanes = processors_per_device(configFile);
+    for (int w = 0; w < devices * lanes; w++) {
+        risa_bp_t h = nullptr;
+        check(risa_hip_bp_create(&g_, w / lanes, &h), "BackprojectionMixed");
+        h_.push_back(h);
+        glados::hip::ScopedDevice sd(w / lanes);
+        int variant = 0;
+        if (w % lanes == 0) check(risa_hip_bp_autotune_f16(h, 1, nullptr), "BackprojectionMixed autotune");
+        check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "BackprojectionMixed variant");
+        check(risa_hip_bp_set_variant(h, variant), "BackprojectionMixed variant");
+    }
+    start("BackprojectionMixed", devices, lanes, pool, img_elems(g_),
+          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
+              check(risa_hip_bp_run_f16(h_[(std::size_t)w], in.data(), kMixedSinoScale, 1, out.data(), s),
+                    "BackprojectionMixed::processor");
+          });
+}
+
+BackprojectionMixed::~BackprojectionMixed()
+{
+    stop();
+    for (auto h : h_) risa_hip_bp_destroy(h);
+}
+
 // ------------------------------------------------------- Backprojection --
 Backprojection::Backprojection(const std::string& configFile)
 {
@@ -340,7 +413,7 @@ Masking::~Masking()
 }
 
 // ------------------------------------------------------------------ D2H --
-D2H::D2H(const std::string& configFile)
+D2H::D2H(const std::string& configFile) : rate_((std::size_t)std::max(1, config_int_or(configFile, "rateInterval", 10000)))
 {
     const risa_geom g = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_D2H");
@@ -351,7 +424,18 @@ D2H::D2H(const std::string& configFile)
                   "hipMemcpyAsync");
     });
 }
-D2H::~D2H() { stop(); }
+D2H::~D2H()
+{
+    stop();
+    std::fprintf(stderr, "D2H: Reconstructed %g Images/s in average (%zu images, %g images/s over the run).\n",
+                 rate_.reconstruction_rate(), rate_.images(), rate_.images_per_second());
+}
+
+void D2H::process(input_type&& img)
+{
+    if (img.valid()) rate_.arrived(img.index());
+    DeviceStage::process(std::move(img));
+}
 
 // ---------------------------------------------------------- OfflineSaver --
 static std::string config_string(const std::string& path, const char* key)

@@ -23,6 +23,7 @@
 #include <glados/Image.h>
 #include <glados/hip/Memory.h>
 
+#include "Accounting.h"
 #include "DeviceStage.h"
 #include "risa_hip.h"
 
@@ -50,10 +51,22 @@ std::vector<uint16_t> reference_frames(const std::string& configFile, const risa
 std::vector<uint16_t> read_module_frames(const std::string& stem, const risa_geom& g, int frames);
 void write_module_frames(const std::string& stem, const risa_geom& g, int frames, const std::vector<uint16_t>& data);
 
+// Copies host frames to the devices, dealt round-robin by index
+// (H2D.cu:84-169).  process() counts the indices missing between arrivals
+// (H2D.cu:102-107); the destructor logs the count and the inter-arrival
+// extremes as the reference does (:80-81).
 class H2D : public DeviceStage<HostImage<uint16_t>, DeviceImage<uint16_t>> {
 public:
     explicit H2D(const std::string& configFile);
     ~H2D();
+    void process(input_type&& img);
+    long long lostFrames() const { return loss_.lost(); }
+    double lossPercent() const { return loss_.loss_percent(); }
+    double bestCaseTime() const { return loss_.best_case_seconds(); }
+    double worstCaseTime() const { return loss_.worst_case_seconds(); }
+
+private:
+    FrameLossCounter loss_;
 };
 
 // Permutes each ROFEX frame from its module-major layout (27 modules x P_f x
@@ -100,6 +113,36 @@ private:
     std::vector<risa_filter_t> h_;
 };
 
+// The mixed-precision form of Filter (selected by precision = "mixed" in the
+// configuration): the DFT-GEMM on the matrix cores (FP16 hi + lo split, FP32
+// accumulation) writing each filtered sinogram as FP16 [P][D], value *
+// kMixedSinoScale.  Allocating (pool memPoolSize_filter, default 16).
+constexpr float kMixedSinoScale = 256.0f;
+class FilterMixed : public DeviceStage<DeviceImage<float>, DeviceImage<uint16_t>> {
+public:
+    explicit FilterMixed(const std::string& configFile);
+    ~FilterMixed();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_filter_t> h_;
+};
+
+// Backprojection of FilterMixed's FP16 sinograms with FP32 accumulation
+// (risa_hip_bp_run_f16); masking fused as in Backprojection.
+class BackprojectionMixed : public DeviceStage<DeviceImage<uint16_t>, DeviceImage<float>> {
+public:
+    explicit BackprojectionMixed(const std::string& configFile);
+    ~BackprojectionMixed();
+
+private:
+    risa_geom g_{};
+    std::vector<risa_bp_t> h_;
+};
+
+// precision = "mixed" in the configuration (default "fp32").
+bool mixed_precision(const std::string& configFile);
+
 class Backprojection : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
 public:
     explicit Backprojection(const std::string& configFile);
@@ -120,10 +163,21 @@ private:
     std::vector<risa_mask_t> h_;
 };
 
+// Copies slices back to the host (D2H.cu:83-148).  process() keeps the
+// reference's reconstruction rate: a running mean of the rate over windows of
+// rateInterval images (optional key, default 10000 as D2H.cu:85), logged at
+// destruction (:42); imagesPerSecond() is the whole-run rate.
 class D2H : public DeviceStage<DeviceImage<float>, HostImage<float>> {
 public:
     explicit D2H(const std::string& configFile);
     ~D2H();
+    void process(input_type&& img);
+    double reconstructionRate() const { return rate_.reconstruction_rate(); }
+    double imagesPerSecond() const { return rate_.images_per_second(); }
+    std::size_t images() const { return rate_.images(); }
+
+private:
+    RateMeter rate_;
 };
 
 // Saver for glados::pipeline::SinkStage (OfflineSaver.cu:66-178), keys

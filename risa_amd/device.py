@@ -327,3 +327,8 @@ class Chain(_Handle):
 
     def set_input_layout(self, layout: int):
         check(load().risa_hip_chain_set_input_layout(self.h, layout))
+
+    def graph_count(self) -> int:
+        n = C.c_int()
+        check(load().risa_hip_chain_graph_count(self.h, C.byref(n)))
+        return n.value

@@ -1,0 +1,323 @@
+"""GPU parity of the paths every detector count takes: the filter at lengths
+whose factors are not all 2, 3 and 5 (zero-padded transform of the same
+circular convolution), odd counts in the rebinning fold, C3's 96 uneven
+geometries filtered *and* backprojected, and run calls recorded into a
+hipGraph (they allocate nothing and do not synchronise).  Everything goes
+through the C-ABI and is compared with the oracle on the same inputs."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import disk_mask, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+# 7, 257 and 293 are prime; 261 = 9 * 29; 303 = 3 * 101; 1021 is prime;
+# 2047 = 23 * 89; 2048 and 1000 take the direct transform.
+ANY_D = [7, 257, 261, 293, 303, 511, 1000, 1021, 2047, 2048]
+
+
+@pytest.mark.parametrize("D", ANY_D)
+def test_filter_fft32_any_length(risa, orc, geom_c0, cuda, D):
+    """FFT32 filtering of any D against the FP64 circulant (the reference's
+    unpadded length-D cuFFT convolution, Filter.cu:155-163)."""
+    from risa_amd import device
+
+    print(f"case: FFT32 filter D={D}", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=37, filterType=1)
+    x = (np.random.default_rng(D).standard_normal((2,) + g.par_shape) * 3.0).astype(np.float32)
+    flt = device.Filter(g, 0)
+    t = _t(x, cuda)
+    flt(t, 2)
+    _sync()
+    ref = orc.filter_rows_f64(orc.taps_f32(g).astype(np.float64), x.astype(np.float64))
+    err = rel_rmse(t.cpu().numpy(), ref)
+    print(f"  rel RMSE vs FP64 {err:.2e}", flush=True)
+    assert err < 2e-6
+    flt.close()
+
+
+@pytest.mark.parametrize("D", [7, 257, 261, 293, 303, 1021])
+def test_filter_gemm_and_adjoint_any_length(risa, orc, geom_c0, cuda, D):
+    """GEMM mode (no FFT length condition) in place against the FP64
+    circulant, and the adjoint (the same symmetric operator, through the
+    padded FFT) by the dot-product test."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: GEMM filter and adjoint D={D}", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=29, filterType=2)
+    rng = np.random.default_rng(D + 5)
+    x = (rng.standard_normal((3,) + g.par_shape) * 2.0).astype(np.float32)
+    flt = device.Filter(g, mode=1)
+    t = _t(x, cuda)
+    flt(t, 3)
+    _sync()
+    ref = orc.filter_rows_f64(orc.taps_f64(g), x.astype(np.float64))
+    err = rel_rmse(t.cpu().numpy(), ref)
+    print(f"  GEMM rel RMSE vs FP64 {err:.2e}", flush=True)
+    assert err < 5e-6
+    y = rng.standard_normal((3,) + g.par_shape).astype(np.float32)
+    fx, fty = _t(x, cuda), _t(y, cuda)
+    flt(fx, 3)
+    flt.adjoint(fty, 3)
+    _sync()
+    lhs = float(np.sum(fx.cpu().numpy().astype(np.float64) * y))
+    rhs = float(np.sum(x.astype(np.float64) * fty.cpu().numpy()))
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"  dot test <Fx, y> vs <x, F^T y>: rel gap {gap:.2e}", flush=True)
+    assert gap <= 1e-5
+    flt.close()
+
+
+@pytest.mark.parametrize("D,S,gscale", [(7, 1, 1.0), (257, 2, 1e-6), (293, 1, 1e3), (1021, 1, 1.0), (2047, 1, 1.0)])
+def test_filter_grad_taps_any_length(risa, orc, geom_c0, cuda, D, S, gscale):
+    """Tap gradient for padded lengths (summed cross-correlation, one inverse
+    transform, wrap, cosine transform) against the FP64 oracle's
+    sum_rows Re(conj(G_k) X_k) w_k."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: grad taps D={D} slices={S}", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=41)
+    rng = np.random.default_rng(D * 3 + S)
+    x = (rng.standard_normal((S,) + g.par_shape) * 5.0).astype(np.float32)
+    gr = (rng.standard_normal((S,) + g.par_shape) * gscale).astype(np.float32)
+    flt = device.Filter(g)
+    dt = torch.empty(D // 2 + 1, dtype=torch.float32, device=cuda)
+    flt.grad_taps(_t(x, cuda), _t(gr, cuda), S, dt)
+    _sync()
+    ref = orc.grad_taps_f64(x.astype(np.float64), gr.astype(np.float64))
+    err = float(np.linalg.norm(dt.cpu().numpy() - ref) / np.linalg.norm(ref))
+    print(f"  rel L2 vs FP64 oracle {err:.2e}", flush=True)
+    assert err < 1e-5
+    flt.close()
+
+
+@pytest.mark.parametrize("D", [255, 257])
+def test_fan2para_odd_detectors(risa, orc, geom_c0, frames_c0, cuda, D):
+    """Odd numberOfParallelDetectors: the reference's mirror for odd D
+    (cuda_kernels_fan2para.h:164-169) leaves output D/2 with one addend and
+    gives output D/2 - 1 three; reference-arithmetic weights agree with the
+    faithful FP32 oracle, FP64 weights with the FP64 oracle."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: fan2para odd D={D}", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D)
+    fs = frames_c0
+    mask = orc.area_mask(g)
+    tables = orc.Tables(g, 0)
+    att = orc.attenuation_f32(g, fs.raw[1], 0, fs.avg_ref, fs.avg_dark, mask)
+    ref32 = orc.gather_f32(g, tables, 0, att)
+    ref64 = orc.gather_f64(g, tables, 0, att.astype(np.float64))
+    out = torch.empty(g.par_shape, dtype=torch.float32, device=cuda)
+    device.Fan2Para(g, weights=1)(_t(att, cuda), 0, out)
+    _sync()
+    got = out.cpu().numpy()
+    assert np.array_equal(got == 0, ref32 == 0)
+    assert np.max(np.abs(got - ref32)) <= 2e-6 * np.max(np.abs(ref32))
+    h = D // 2
+    assert np.max(np.abs(got[:, h - 1] - ref32[:, h - 1])) <= 2e-6 * np.max(np.abs(ref32))
+    f2p = device.Fan2Para(g)
+    f2p(_t(att, cuda), 0, out)
+    _sync()
+    got = out.cpu().numpy()
+    err = rel_rmse(got, ref64)
+    print(f"  FP64-weight gather rel RMSE vs FP64 oracle {err:.3e}", flush=True)
+    assert err < 5e-7
+    # a batch over both orders of frames equals the single-frame launches
+    fan = _t(np.stack([att, att[::-1].copy()]), cuda)
+    outb = torch.empty((2,) + g.par_shape, dtype=torch.float32, device=cuda)
+    f2p.batch(fan, [0, 0], outb)
+    for f in range(2):
+        one = torch.empty(g.par_shape, dtype=torch.float32, device=cuda)
+        f2p(fan[f], 0, one)
+        assert torch.equal(one, outb[f])
+
+
+def _uneven_jobs(n=96):
+    Ns, Ps = (256, 384, 512, 768), (360, 500, 720, 1000)
+    rng = np.random.default_rng(96)
+    jobs = []
+    for i in range(n):
+        jobs.append(dict(N=Ns[i % 4], P=Ps[(i // 4) % 4], D=int(256 + (i * 37) % 513),
+                         dtype=(i // 16) % 2, rot=float(rng.uniform(-45, 45)), mask=int(i % 3 != 0),
+                         window=(i // 3) % 4))
+    return jobs
+
+
+def test_c3_filtered_uneven_geometries(risa, orc, geom_c0, cuda):
+    """C3 as filtered backprojection: each of the 96 geometries (256..768
+    detectors, most of them not 2/3/5-smooth) is filtered on the GPU -- FP32
+    jobs by the FFT, FP16 jobs by the DFT-GEMM writing FP16 -- and all 96 are
+    backprojected in one persistent launch.  FP32 jobs are compared with the
+    FP64 oracle's filter + backprojection (rel RMSE <= 1e-5), FP16 jobs within
+    the mixed-precision bound 2e-4."""
+    import torch
+    from risa_amd import device, synth
+
+    specs = _uneven_jobs()
+    scale16 = 256.0
+    jobs, refs = [], []
+    print(f"case: C3 filtered, {len(specs)} jobs", flush=True)
+    for i, s in enumerate(specs):
+        g = geom_c0.replace(numberOfPixels=s["N"], numberOfParallelProjections=s["P"],
+                            numberOfParallelDetectors=s["D"], rotationOffset=s["rot"], fusedMasking=s["mask"],
+                            maskingValue=-1.0, filterType=s["window"])
+        sino = synth.parallel_sinogram(5000 + i, 190.0, s["P"], s["D"], rotation=s["rot"])
+        flt = device.Filter(g, mode=1 if s["dtype"] else 0)
+        if s["dtype"] == 1:
+            out16 = torch.empty((s["P"], s["D"]), dtype=torch.float16, device=cuda)
+            flt.f16(_t(sino, cuda), out16, 1, scale16)
+            dev_sino = out16
+        else:
+            dev_sino = _t(sino, cuda)
+            flt(dev_sino, 1)
+        flt.close()
+        img = torch.full((s["N"], s["N"]), float("nan"), dtype=torch.float32, device=cuda)
+        jobs.append(dict(sino=dev_sino, img=img, N=s["N"], P=s["P"], D=s["D"], dtype=s["dtype"],
+                         in_scale=scale16 if s["dtype"] else 1.0, rotationOffset=s["rot"], mask=s["mask"],
+                         maskingValue=-1.0))
+        filt64 = orc.filter_rows_f64(orc.taps_f64(g), sino.astype(np.float64))
+        ref = orc.bp_linear_f64(g, filt64)
+        if s["mask"]:
+            ref = orc.masking_f64(g.replace(normalization=0), ref)
+        refs.append((g, ref))
+    _sync()
+    bb = device.BatchedBackprojection(jobs)
+    bb.autotune()
+    for j in jobs:
+        j["img"].fill_(float("nan"))
+    bb()
+    _sync()
+    worst = {0: 0.0, 1: 0.0}
+    for i, (s, j, (g, ref)) in enumerate(zip(specs, jobs, refs)):
+        got = j["img"].cpu().numpy()
+        assert np.isfinite(got).all(), i
+        m = disk_mask(s["N"]) if s["mask"] else np.ones((s["N"], s["N"]), bool)
+        if s["mask"]:
+            assert (got[~m] == -1.0).all(), i
+        err = rel_rmse(got, ref, m)
+        worst[s["dtype"]] = max(worst[s["dtype"]], err)
+        assert err < (1e-5 if s["dtype"] == 0 else 2e-4), (i, s, err)
+    print(f"  worst rel RMSE vs FP64 filter+BP: FP32 jobs {worst[0]:.2e}, FP16 jobs {worst[1]:.2e}", flush=True)
+    bb.close()
+
+
+def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
+    """The run calls that used scratch buffers (GEMM-mode in-place filter, tap
+    gradient, FP16 plain-layout backprojection, adjoint) record into a
+    hipGraph: they allocate nothing and do not synchronise.  Each replay --
+    with eager work and synchronisations between replays -- equals the eager
+    result bit for bit."""
+    import torch
+    from risa_amd import device
+
+    g = geom_c0.replace(numberOfParallelDetectors=261, numberOfParallelProjections=120, numberOfPixels=192)
+    P, D, N = g.numberOfParallelProjections, g.numberOfParallelDetectors, g.numberOfPixels
+    rng = np.random.default_rng(123)
+    S = 5
+    x0 = _t((rng.standard_normal((S, P, D)) * 2).astype(np.float32), cuda)
+    gr = _t(rng.standard_normal((S, P, D)).astype(np.float32), cuda)
+    img0 = _t(rng.standard_normal((N, N)).astype(np.float32), cuda)
+    S16 = 18                                    # two passes through the 16-slice staging buffer
+    h16 = _t((rng.standard_normal((S16, P, D)) * 100).astype(np.float16), cuda)
+    fg = device.Filter(g, mode=1)
+    ff = device.Filter(g, mode=0)
+    bp = device.Backprojection(g)
+
+    def work(o):
+        o["x"].copy_(x0)
+        fg(o["x"], S)                           # GEMM mode, in place through the staging buffer
+        ff.grad_taps(x0, gr, S, o["dt"])        # padded-length tap gradient
+        bp.f16(h16, 256.0, S16, o["img16"])     # plain-layout FP16 through the staging buffer
+        bp.adjoint(img0, o["adj"])              # transposed image buffer
+
+    def outs():
+        return dict(x=torch.empty_like(x0), dt=torch.empty(D // 2 + 1, device=cuda),
+                    img16=torch.empty((S16, N, N), device=cuda), adj=torch.empty((P, D), device=cuda))
+
+    print("case: run calls under graph capture (GEMM filter, grad taps, FP16 BP, adjoint)", flush=True)
+    eager = outs()
+    work(eager)
+    _sync()
+    cap = outs()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        work(cap)
+    for rep in range(3):
+        for v in cap.values():
+            v.fill_(float("nan"))
+        _sync()
+        graph.replay()
+        _sync()
+        for k in eager:
+            assert torch.equal(cap[k], eager[k]), (rep, k)
+        work(outs())                            # eager work between replays
+        _sync()
+    del graph
+    for h in (fg, ff, bp):
+        h.close()
+
+
+def test_chain_graph_cache_bound_and_recapture(risa, orc, geom_c0, frames_c0, cuda):
+    """The chain keeps at most 128 graph executables (least recently replayed
+    released after a device synchronisation); changing the variant or the
+    input layout after capture drains the device, releases the graphs and the
+    next graph run captures anew, with results equal to eager runs."""
+    import torch
+    from risa_amd import device
+
+    g, fs = geom_c0, frames_c0
+    raw = _t(fs.raw[:1], cuda)
+    planes = fs.planes[:1]
+    ch = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_PLAIN, max_frames=1)
+    want = torch.empty((1,) + g.img_shape, dtype=torch.float32, device=cuda)
+    ch(raw, planes, want)
+    _sync()
+    n_img = 136
+    imgs = torch.full((n_img,) + g.img_shape, float("nan"), dtype=torch.float32, device=cuda)
+    s = torch.cuda.Stream()
+    print(f"case: chain graph cache, {n_img} buffer sets", flush=True)
+    with torch.cuda.stream(s):
+        for k in range(n_img):
+            ch(raw, planes, imgs[k:k + 1], use_graph=True)
+        for k in range(4):                      # replays of evicted and kept shapes
+            ch(raw, planes, imgs[k:k + 1], use_graph=True)
+    s.synchronize()
+    assert ch.graph_count() <= 128
+    for k in range(n_img):
+        assert torch.equal(imgs[k], want[0]), k
+    print("case: chain variant change after capture", flush=True)
+    v0 = ch.variant
+    others = [v for v in (0, 1, 2, 3) if v != (v0 & 0xFF)]
+    for v in others[:3]:
+        ch.variant = (v0 & ~0xFF) | v
+        assert ch.graph_count() == 0
+        eager = torch.empty_like(want)
+        ch(raw, planes, eager)
+        out = torch.full_like(want, float("nan"))
+        with torch.cuda.stream(s):
+            ch(raw, planes, out, use_graph=True)
+        s.synchronize()
+        _sync()
+        assert ch.graph_count() == 1
+        assert torch.equal(out, eager), v
+    ch.close()

@@ -39,16 +39,28 @@ def read_saved(path, N):
     return out
 
 
-@pytest.mark.parametrize("cfg,extra", [("c0", ()), ("c1", ("--masking",)), ("c0", ("--module-major",))])
-def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extra):
+@pytest.mark.parametrize("cfg,extra,precision", [("c0", (), "fp32"), ("c1", ("--masking",), "fp32"),
+                                                 ("c0", ("--module-major",), "fp32"), ("c0", (), "mixed"),
+                                                 ("c1", (), "mixed")])
+def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extra, precision):
+    """precision = "mixed" wires FilterMixed (DFT-GEMM on MFMA, FP16 sinogram)
+    and BackprojectionMixed (FP16 sinogram, FP32 accumulation): within the
+    mixed-precision bound 2e-4; the FP32 stages within 1e-5."""
     from risa_amd import synth
 
-    g = risa.geom_from_config(risa.config_path(cfg))
+    path = risa.config_path(cfg)
+    if precision != "fp32":
+        path = tmp_path / f"{cfg}_{precision}.cfg"
+        path.write_text(risa.config_path(cfg).read_text() + f'\nprecision = "{precision}"\n')
+    g = risa.geom_from_config(path)
     N, planes = g.numberOfPixels, g.numberOfPlanes
     frames = 24 if cfg == "c0" else 16
     out = tmp_path / "slices.bin"
-    res = run_pipeline(risa.config_path(cfg), frames, 6, out, extra)
+    res = run_pipeline(path, frames, 6, out, extra)
     assert res["frames"] == frames
+    assert res["precision"] == precision
+    assert res["lost_frames"] == 0
+    tol = 1e-5 if precision == "fp32" else 2e-4
     saved = read_saved(out, N)
     assert len(saved) == 6
     # the stage's reference average: R synthetic flat fields, seed 901 (Attenuation(configFile))
@@ -60,10 +72,30 @@ def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extr
         raw = synth.fan_frame(g, k, 1000 + u, 55 + u * planes + k)
         ref = oc.f64(raw, k)
         err = rel_rmse(img, ref, m)
-        print(f"  {cfg} slice {idx} (frame {u}, plane {k}): rel RMSE {err:.2e}", flush=True)
-        assert err < 1e-5
+        print(f"  {cfg} {precision} slice {idx} (frame {u}, plane {k}): rel RMSE {err:.2e}", flush=True)
+        assert err < tol
         assert (img[~m] == g.maskingValue).all()
     oc.close()
+
+
+def test_stage_pipeline_counts_lost_frames(risa, cuda, tmp_path):
+    """A source that leaves out every 5th index (frames lost before the GPU):
+    H2D counts the missing indices and the loss against the last index
+    (H2D.cu:102-107), D2H reports its rate (D2H.cu:85-93); every emitted
+    frame is reconstructed."""
+    frames, k = 40, 5
+    res = run_pipeline(risa.config_path("c0"), frames, 0, tmp_path / "unused.bin", ("--skip-every", str(k)))
+    # emitted indices: 0, 1, 2, 3, 5, ... (i % 5 == 4 left out)
+    idx, i = [], 0
+    while len(idx) < frames:
+        if i % k != k - 1:
+            idx.append(i)
+        i += 1
+    lost = idx[-1] + 1 - len(idx)
+    assert res["frames"] == frames
+    assert res["lost_frames"] == lost
+    assert abs(res["loss_percent"] - lost / idx[-1] * 100.0) < 1e-3
+    assert res["d2h_images_per_s"] > 0
 
 
 @pytest.mark.parametrize("mode", ["offline", "online"])

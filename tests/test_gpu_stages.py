@@ -113,8 +113,9 @@ def test_reorder_and_module_major_attenuation(risa, orc, cfg, cuda):
 def test_chain_module_major_input(risa, orc, geom_c0, frames_c0, cuda):
     """The chain reading ROFEX module-major frames (the reference's H2D ->
     Reordering -> Attenuation, the reordering fused) gives the images of the
-    detector-major chain bit for bit: plain and mixed, eager and graph; the
-    layout is refused once graphs are captured."""
+    detector-major chain bit for bit: plain and mixed, eager and graph;
+    switching the layout back after capture releases the graphs and the next
+    graph run reads detector-major frames again."""
     import torch
     from risa_amd import _lib, device
 
@@ -141,8 +142,16 @@ def test_chain_module_major_input(risa, orc, geom_c0, frames_c0, cuda):
                 b(raw_mm, fs.planes, ib, use_graph=use_graph)
             _sync()
             assert torch.equal(ia, ib)
-        with pytest.raises(_lib.RisaError, match="RISA_ERR_UNSUPPORTED"):
-            b.set_input_layout(risa.LAYOUT_DETECTOR_MAJOR)
+        assert b.graph_count() == 1
+        b.set_input_layout(risa.LAYOUT_DETECTOR_MAJOR)
+        assert b.graph_count() == 0
+        ic = torch.full_like(ia, float("nan"))
+        with torch.cuda.stream(s):
+            b(raw, fs.planes, ic, use_graph=True)
+        _sync()
+        assert torch.equal(ia, ic)
+        a.close()
+        b.close()
 
 
 def test_defect_repair_matches_reference_vectors(risa, cuda):
@@ -417,8 +426,9 @@ def test_chain_plain_matches_fp64_oracle(risa, orc, geom_c0, frames_c0, cuda):
 
 def test_chain_variant_override(risa, orc, geom_c0, frames_c0, cuda):
     """The chain's backprojector variant can replace the autotuner's pick
-    before the first graph run (each tile shape stays within the plain
-    chain's tolerance) and is refused once graphs hold the old launches."""
+    (each tile shape stays within the plain chain's tolerance), also after
+    graphs were captured: the change releases them and the next graph run
+    captures the new shape."""
     import torch
     from risa_amd import _lib, device
 
@@ -441,8 +451,16 @@ def test_chain_variant_override(risa, orc, geom_c0, frames_c0, cuda):
         got = img.cpu().numpy()
         for f in range(2):
             assert rel_rmse(got[f], refs[f], m) < 1e-5, (v, f)
-        with pytest.raises(_lib.RisaError):
-            ch.variant = v
+        w = 2 - v
+        ch.variant = (ch.variant & ~0xFF) | w
+        assert ch.graph_count() == 0
+        img2 = torch.full_like(img, float("nan"))
+        with torch.cuda.stream(s):
+            ch(raw, planes, img2, use_graph=True)
+        s.synchronize()
+        got = img2.cpu().numpy()
+        for f in range(2):
+            assert rel_rmse(got[f], refs[f], m) < 1e-5, (w, f)
         ch.close()
 
 
@@ -540,7 +558,7 @@ def test_filter_gemm_split_matches_fp64(risa, orc, geom_c0, cuda, D, P, slices):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("slices", [1, 4, 6])
+@pytest.mark.parametrize("slices", [1, 4, 6, 18])
 def test_backprojection_f16_matches_oracle(risa, orc, geom_c1, cuda, slices):
     """FP16-sinogram / FP32-accumulate backprojection: the plain-layout and
     x4-layout entry points give identical images, within the FP16 weight
