@@ -236,6 +236,56 @@ def test_ellipse_map_fixed_points_and_identity(orc):
         assert abs(orc.ellipse_f64(a, 0.0, 1417.0, 365.0) - a) < 1e-7
 
 
+def ellipse_from_geometry(alpha, dx, dz, diam):
+    """The tilted-ring map built from its geometry, not from the reference's
+    text: the source ring point S(a) = (R cos a, R sin a, 0), the tilt centre
+    Q = (-dX, 0, -dZ); eps(a) is the 3-D angle at Q between Q->O and Q->S(a)
+    (atan2 of |cross| and dot), gamma = (eps(pi) - eps(0)) / 2, and the map is
+    the polar angle of (eps cos a + gamma, eps sin a) in [0, 2 pi)."""
+    a = np.asarray(alpha, np.float64)
+    R = diam / 2.0
+    u = np.array([dx, 0.0, dz])                                        # Q -> O
+
+    def eps(al):
+        v = np.stack([R * np.cos(al) + dx, R * np.sin(al), np.full_like(al, dz)], -1)   # Q -> S
+        cr = np.cross(np.broadcast_to(u, v.shape), v)
+        return np.arctan2(np.linalg.norm(cr, axis=-1), v @ u)
+
+    e = eps(a)
+    gam = 0.5 * (eps(np.array([math.pi]))[0] - eps(np.array([0.0]))[0])
+    out = np.arctan2(e * np.sin(a), e * np.cos(a) + gam)
+    return np.where(out < 0, out + 2 * math.pi, out)
+
+
+@pytest.mark.parametrize("dx,dz,diam", [(815.0, 1417.0, 365.0), (815.0, 1430.0, 370.0), (300.0, 900.0, 500.0),
+                                        (-400.0, 1200.0, 300.0)])
+def test_ellipse_map_matches_its_geometry(orc, dx, dz, diam):
+    """orc_ellipse_kreis_f64 / _f32 (cuda_kernels_fan2para.h:38-62) against the
+    map constructed from the tilted-ring geometry.  A sign slip in eps's
+    numerator (L^2 + R dX cos a), in gamma's order, or in the (eps cos a +
+    gamma, eps sin a) vector moves the result by far more than the bounds.
+    FP64: 1e-12 rad where |sin a| > 0.01; acos near +-1 (outputs near 0 and
+    pi) loses up to sqrt(2 eps) ~ 2e-8 there.  FP32 (both libm bindings):
+    2e-5 rad where |sin a| > 0.05, 1e-3 everywhere (float acos near +-1)."""
+    a = np.concatenate([np.linspace(1e-4, 2 * math.pi - 1e-4, 2501), [math.pi / 2, math.pi, 1.5 * math.pi]])
+    want = ellipse_from_geometry(a, dx, dz, diam)
+    got = np.array([orc.ellipse_f64(x, dx, dz, diam) for x in a])
+    away = np.abs(np.sin(a)) > 0.01
+    assert np.abs(got - want)[away].max() < 1e-12
+    assert np.abs(got - want).max() < 1e-7
+    far = np.abs(np.sin(a)) > 0.05
+    for variant in (0, 1):
+        g32 = np.array([orc.ellipse_f32(np.float32(x), dx, dz, diam, variant) for x in a], np.float64)
+        w32 = ellipse_from_geometry(a.astype(np.float32).astype(np.float64), dx, dz, diam)
+        err = np.abs(g32 - w32)
+        err = np.minimum(err, 2 * math.pi - err)
+        assert err[far].max() < 2e-5, variant
+        assert err.max() < 1e-3, variant
+    # the map is not the identity under tilt (the invariants alone would pass that)
+    if dx != 0:
+        assert np.abs(want - np.where(a < 2 * math.pi, a, a - 2 * math.pi)).max() > 1e-3
+
+
 def small_geom(risa, **kw):
     base = dict(numberOfFanDetectors=96, numberOfFanProjections=120, numberOfParallelProjections=60,
                 numberOfParallelDetectors=64, numberOfPlanes=2)
@@ -376,6 +426,37 @@ def test_rebin_tables_product_host_bit_exact(orc, risa, geom_c0):
             for i, name in enumerate(orc.Tables.FLOAT_NAMES):
                 np.testing.assert_array_equal(flts[i], tab.plane(name, k), err_msg=name)
         tab.close()
+
+
+def test_rebin_tables_product_host_bit_exact_c1_both_planes(orc, risa, geom_c1):
+    """The production geometry (768 -> 512 detectors, 2000 internal angles,
+    two tilted planes): the product's binary-search tables equal the
+    oracle's literal scan entry for entry on both planes.  Also counts the
+    integer entries that change when the reference's unqualified libm calls
+    bind to the double overloads (the one choice only nvcc could settle)."""
+    from risa_amd import _lib
+
+    lib = _lib.load()
+    g = geom_c1
+    tab = orc.Tables(g, 0)
+    tab64 = orc.Tables(g, 1)
+    n = tab.t.P2 * tab.t.D
+    differing = []
+    for k in range(g.numberOfPlanes):
+        ints = np.empty((10, n), np.int32)
+        flts = np.empty((4, n), np.float32)
+        _lib.check(lib.risa_fan2para_tables(g, k, _lib.ptr(ints), _lib.ptr(flts)))
+        for i, name in enumerate(orc.Tables.INT_NAMES):
+            np.testing.assert_array_equal(ints[i], tab.plane(name, k), err_msg=f"{name} plane {k}")
+        for i, name in enumerate(orc.Tables.FLOAT_NAMES):
+            np.testing.assert_array_equal(flts[i], tab.plane(name, k), err_msg=f"{name} plane {k}")
+        assert tab.plane("ray1", k).sum() > 0 and tab.plane("ray2", k).sum() > 0
+        differing.append(int(sum((tab.plane(nm, k) != tab64.plane(nm, k)).sum() for nm in orc.Tables.INT_NAMES)))
+    print(f"C1 integer entries changed by the double libm binding: plane 0 {differing[0]}, plane 1 {differing[1]} "
+          f"of {10 * n} each")
+    assert max(differing) < 1e-4 * 10 * n
+    tab.close()
+    tab64.close()
 
 
 # ------------------------------------------------------------------ a6 gather --
@@ -570,6 +651,110 @@ def test_bp_brute_force_tiny(orc, risa):
             exp[yy, xx] = acc * norm
     np.testing.assert_allclose(orc.bp_linear_f64(g, sino), exp, rtol=1e-12, atol=1e-12)
     assert rel_rmse(orc.bp_linear_f32(g, sino.astype(np.float32)), exp) < 1e-6
+
+
+def _f32_round(fr):
+    """Round an exact rational to the nearest float32 (ties to even), the way
+    an IEEE single-precision operation rounds its exact result."""
+    from fractions import Fraction
+
+    if fr == 0:
+        return 0.0
+    sign = -1 if fr < 0 else 1
+    m = abs(Fraction(fr))
+    e = m.numerator.bit_length() - m.denominator.bit_length()
+    if Fraction(2) ** e > m:
+        e -= 1
+    q = m / Fraction(2) ** (e - 23)                  # in [2^23, 2^24)
+    n, r = divmod(q.numerator, q.denominator)
+    if 2 * r > q.denominator or (2 * r == q.denominator and n % 2 == 1):
+        n += 1
+    return sign * float(Fraction(n) * Fraction(2) ** (e - 23))
+
+
+def _f32_ops():
+    from fractions import Fraction
+
+    F = Fraction
+    mul = lambda a, b: _f32_round(F(a) * F(b))               # noqa: E731
+    add = lambda a, b: _f32_round(F(a) + F(b))               # noqa: E731
+    fma = lambda a, b, c: _f32_round(F(a) * F(b) + F(c))     # noqa: E731
+    return mul, add, fma
+
+
+def _round_half_away(v):
+    return math.floor(v + 0.5) if v >= 0 else math.ceil(v - 0.5)
+
+
+@pytest.mark.parametrize("N,P,D,rot", [(7, 11, 5, 0.0), (8, 6, 9, 17.0), (6, 13, 4, -40.0), (9, 7, 8, 90.0),
+                                       (8, 4, 8, 0.0)])
+def test_bp_nearest_and_tex_brute_force(orc, risa, N, P, D, rot):
+    """backProjectNearest and backProjectTex (Backprojection.cu:273-336) by
+    per-pixel brute force in pure Python with every float32 operation rounded
+    exactly (rational arithmetic): nearest t = round-half-away-from-zero of
+    xp cos + yp sin (the multiply-add contracted as fma(yp, sin, xp cos), the
+    oracle's stated reading of nvcc's default contraction) plus floor(D/2),
+    xp = (x - (N-1)/2) / (N/D) in double, result sum * pi / P * (N/D) in
+    double; tex t = int() truncation toward zero of xp cos + fma(yp, sin, D/2),
+    xp = (x - c) * (D/N) in float, result sum * norm.  Bit for bit; truncation
+    vs floor, half-to-even rounding, a swapped scale or a missing bounds test
+    each change pixels here (t reaches -D/2-1 .. D/2+1)."""
+    g = rofex_geom(risa, numberOfParallelDetectors=D, numberOfPixels=N, numberOfParallelProjections=P,
+                   rotationOffset=rot)
+    s, c, k = orc.bp_lut_f32(g)
+    rng = np.random.default_rng(N * 100 + P * 10 + D)
+    sino = rng.standard_normal((P, D)).astype(np.float32)
+    mul, add, fma = _f32_ops()
+    f = lambda v: float(np.float32(v))                      # noqa: E731
+    near = np.zeros((N, N), np.float32)
+    tex = np.zeros((N, N), np.float32)
+    from fractions import Fraction
+
+    scale_n = _f32_round(Fraction(N) / Fraction(D))         # (float)N / (float)D
+    cI_int = int(D * 0.5)
+    cI_f = f(D * 0.5)
+    for y in range(N):
+        for x in range(N):
+            # nearest
+            xp = f((x - (N - 1.0) * 0.5) / scale_n)
+            yp = f((y - (N - 1.0) * 0.5) / scale_n)
+            acc = 0.0
+            for p in range(P):
+                tt = fma(yp, float(s[p]), mul(xp, float(c[p])))
+                t = int(add(float(_round_half_away(tt)), float(cI_int)))
+                if 0 <= t < D:
+                    acc = add(acc, float(sino[p, t]))
+            near[y, x] = f(acc * math.pi / P * scale_n)
+            # tex
+            xq = mul(add(float(x), -float(k[2])), float(k[0]))
+            yq = mul(add(float(y), -float(k[2])), float(k[0]))
+            acc = 0.0
+            for p in range(P):
+                tt = add(mul(xq, float(c[p])), fma(yq, float(s[p]), cI_f))
+                t = int(tt)
+                if 0 <= t < D:
+                    acc = add(acc, float(sino[p, t]))
+            tex[y, x] = mul(acc, float(k[1]))
+    np.testing.assert_array_equal(orc.bp_nearest_f32(g, sino), near)
+    np.testing.assert_array_equal(orc.bp_tex_f32(g, sino), tex)
+
+
+def test_bp_nearest_and_tex_constant_sinogram(orc, risa):
+    """A sinogram constant at c: every in-range sample adds c, so wherever the
+    picked sample stays inside the detector for all angles both nearest forms
+    give P c * pi / P * N/D = c pi N/D (to float rounding of the sum)."""
+    for D, N, P in ((64, 64, 90), (96, 64, 50), (64, 80, 33)):
+        g = rofex_geom(risa, numberOfParallelDetectors=D, numberOfPixels=N, numberOfParallelProjections=P)
+        cval = 0.75
+        sino = np.full((P, D), cval, np.float32)
+        scale = D / N
+        y, x = np.mgrid[0:N, 0:N]
+        r = np.hypot((x - (N - 1) / 2) * scale, (y - (N - 1) / 2) * scale)
+        inner = r < D // 2 - 2
+        for img in (orc.bp_nearest_f32(g, sino), orc.bp_tex_f32(g, sino)):
+            np.testing.assert_allclose(img[inner], cval * math.pi * N / D, rtol=2e-6)
+            # outside the detector's reach no sample is picked at all for some angle
+            assert (img[r > D / 2 + 2] < cval * math.pi * N / D * (1 - 0.5 / P)).all()
 
 
 def test_bp_adjoint_dot_product(orc, risa):
