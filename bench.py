@@ -97,6 +97,17 @@ def kernel_only(args, g, dev):
                 img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
                 fn, per = (lambda: bp.f16x4(sino, 256.0, B, img)), B
                 what = f"bp FP16 x4 layout, {B} slices {N}^2 x {P} per launch, variant {bp.variant >> 8}"
+        elif args.kernel == "adj":
+            # the differentiable FBP's backward projector at its C4 shape:
+            # 16 slices of 512^2, 720 angles, 512 detectors per batched call
+            ga = g.replace(numberOfParallelDetectors=512, numberOfParallelProjections=720, numberOfPixels=512,
+                           filterType=0, fusedMasking=1)
+            S4, P4, D4, N4 = 16, 720, 512, 512
+            bpa = device.Backprojection(ga)
+            im4 = torch.from_numpy(rng.standard_normal((S4, N4, N4)).astype(np.float32)).to(dev)
+            gs4 = torch.empty((S4, P4, D4), dtype=torch.float32, device=dev)
+            fn, per = (lambda: bpa.adjoint_batch(im4, S4, gs4)), S4
+            what = f"adjoint projector, {S4} slices of {N4}^2 -> {P4} x {D4} per call"
         elif args.kernel in ("att", "att_mm", "reorder"):
             # 32 C1 frames per launch: attenuation of detector-major frames,
             # the same reading ROFEX module-major frames through the fused
@@ -168,8 +179,10 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the streamed C++ stage pipeline leg")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch (96 geometries) and differentiable-FBP legs")
-    ap.add_argument("--stream-frames", type=int, default=2000, help="frames through the stage pipeline")
-    ap.add_argument("--kernel", choices=("bp", "bp16", "att", "att_mm", "reorder", "prep", "copy"),
+    ap.add_argument("--stream-frames", type=int, default=4000,
+                    help="frame-planes through the stage pipeline (C1: 2 planes x 2000 frames)")
+    ap.add_argument("--no-c0", action="store_true", help="skip the C0 GPU / CPU leg")
+    ap.add_argument("--kernel", choices=("bp", "bp16", "adj", "att", "att_mm", "reorder", "prep", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     args = ap.parse_args()
@@ -382,12 +395,10 @@ def main():
             def fwd():
                 y4.copy_(x4)
                 filt4(y4, S4)
-                for s_ in range(S4):
-                    bp4(y4[s_], im4[s_])
+                bp4.batch(y4, S4, im4)
 
             def bwd():
-                for s_ in range(S4):
-                    bp4.adjoint(im4[s_], gs4[s_])
+                bp4.adjoint_batch(im4, S4, gs4)
                 filt4.grad_taps(x4, gs4, S4, dt4)
                 filt4.adjoint(gs4, S4)
 
@@ -405,31 +416,89 @@ def main():
             adjoint = {"slices": S4, "forward_ms": round(times["forward"], 3), "backward_ms": round(times["backward"], 3),
                        "forward_us_per_slice": round(times["forward"] * 1e3 / S4, 1),
                        "backward_us_per_slice": round(times["backward"] * 1e3 / S4, 1),
-                       "what": "C4 geometry; backward = bp_adjoint_kernel per slice + fft_grad_taps_kernel + "
+                       "what": "C4 geometry; forward = filter + batched backprojection; backward = "
+                               "bp_adjoint_path_kernel (batched, 2 slices per pass) + fft_grad_taps_kernel + "
                                "conjugate filter; one step of the autograd function"}
             log(f"differentiable FBP: {adjoint}")
         except Exception as exc:  # reported, not fatal
             log(f"differentiable FBP timing skipped: {exc}")
 
-    # PCIe-inclusive rate: the C++ stage pipeline (H2D -> ... -> D2H) streaming
-    # pinned host frames; reported beside the device-resident value, never as it.
+    # C1 as its definition reads: every frame streamed from pinned host memory
+    # through the C++ stage queue (H2D -> ... -> D2H), all 2 x 2000 frames, on
+    # device 0 only (the child sees one device); plain FP32 stages and the
+    # mixed-precision stages (precision = "mixed").  PCIe-inclusive, reported
+    # beside the device-resident value, never as it.
     streamed = None
     exe = ROOT / "risa_amd" / "bin" / "risa_pipeline"
     if exe.exists() and not args.no_stream:
         import subprocess
+        import tempfile
 
-        try:
-            r = subprocess.run([str(exe), str(risa_amd.config_path("c1")), "--frames", str(args.stream_frames),
-                                "--unique", "16"], capture_output=True, text=True, timeout=300)
-            if r.returncode == 0:
-                streamed = json.loads(r.stdout.strip().splitlines()[-1])
-                streamed["what"] = ("C++ glados-style stages, frames streamed from pinned host memory "
-                                    "through H2D .. D2H, processorsPerDevice streams per stage")
-            else:
-                log(f"stage pipeline failed: {r.stderr[-500:]}")
-        except Exception as exc:  # reported, not fatal
-            log(f"stage pipeline skipped: {exc}")
+        streamed = {}
+        env = dict(os.environ, HIP_VISIBLE_DEVICES="0")
+        base_cfg = risa_amd.config_path("c1")
+        with tempfile.TemporaryDirectory() as tmp:
+            mixed_cfg = Path(tmp) / "c1_mixed.cfg"
+            mixed_cfg.write_text(base_cfg.read_text() + '\nprecision = "mixed"\n')
+            for label, cfg in (("fp32", base_cfg), ("mixed", mixed_cfg)):
+                try:
+                    r = subprocess.run([str(exe), str(cfg), "--frames", str(args.stream_frames), "--unique", "16"],
+                                       capture_output=True, text=True, timeout=300, env=env)
+                    if r.returncode == 0:
+                        res = json.loads(r.stdout.strip().splitlines()[-1])
+                        res["what"] = ("C++ glados-style stages on device 0, frames streamed from pinned host "
+                                       "memory through H2D .. D2H, processorsPerDevice streams per stage")
+                        streamed[label] = res
+                    else:
+                        log(f"stage pipeline ({label}) failed: {r.stderr[-500:]}")
+                except Exception as exc:  # reported, not fatal
+                    log(f"stage pipeline ({label}) skipped: {exc}")
         log(f"streamed stage pipeline: {streamed}")
+
+    # C0: the reference's own CPU-runnable case (1 plane, 100 frames, 432 x 500
+    # fan -> 256 x 500 -> 256^2, ramp): the GPU plain chain and the
+    # faithful-FP32 CPU chain on all 100 frames.
+    c0 = None
+    if not args.no_c0:
+        try:
+            g0 = risa_amd.geom_from_config(risa_amd.config_path("c0"))
+            F0 = 100
+            ref0 = synth.reference_frames(g0, 8, seed=900)
+            ar0, ad0 = synth.prepare_reference(g0, ref0)
+            raw0_np = np.stack([synth.fan_frame(g0, 0, f, 55 + f) for f in range(F0)])
+            pl0 = np.zeros(F0, np.int32)
+            raw0 = torch.from_numpy(raw0_np).to(dev)
+            img0 = torch.empty((F0,) + g0.img_shape, dtype=torch.float32, device=dev)
+            ch0 = device.Chain(g0, ar0, ad0, mode=risa_amd.CHAIN_PLAIN, max_frames=F0)
+            with torch.cuda.stream(stream):
+                for _ in range(2):
+                    ch0(raw0, pl0, img0, use_graph=True)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(10):
+                    ch0(raw0, pl0, img0, use_graph=True)
+                e1.record(stream)
+            stream.synchronize()
+            ms0 = e0.elapsed_time(e1) / 10
+            ch0.close()
+            c0 = {"frames": F0, "gpu_ms": round(ms0, 3), "gpu_slices_per_s": round(F0 / (ms0 * 1e-3), 1),
+                  "what": "C0 plain FP32 chain, 100 frames per call (one graph), inputs in HBM"}
+            if not args.no_cpu:
+                from oracle import oracle as orc
+
+                oc0 = orc.Chain(g0, ar0, ad0)
+                threads0 = min(16, os.cpu_count() or 1)
+                oc0.f32(raw0_np[:1], pl0[:1], threads=1)
+                t = time.perf_counter()
+                oc0.f32(raw0_np, pl0, threads=threads0)
+                dt0 = time.perf_counter() - t
+                oc0.close()
+                c0.update({"cpu_slices_per_s": round(F0 / dt0, 2), "cpu_cores": threads0, "cpu_kind": "port",
+                           "cpu_seconds": round(dt0, 3),
+                           "gpu_vs_cpu": round((F0 / (ms0 * 1e-3)) / (F0 / dt0), 1)})
+            log(f"C0: {c0}")
+        except Exception as exc:  # reported, not fatal
+            log(f"C0 leg skipped: {exc}")
 
     cpu = None
     if not args.no_cpu:
@@ -494,7 +563,7 @@ def main():
         "mixed": {"value": round(mixed_value, 1), "unit": "slices/s", "ms_per_step": round(ms_mixed, 3),
                   "vs_plain": round(mixed_value / plain_value, 3), "dtype": "f16 storage / f32 accumulate"},
         "accuracy_vs_fp64_oracle": rmse,
-        "roofline": {"bound": "mfma", "resource": "fp32 vector ALU (backprojection is compute-bound; "
+        "roofline": {"bound": "valu", "resource": "fp32 vector ALU (backprojection is compute-bound; "
                                                   "AI ~850 flop/B >> ridge ~20)",
                      "kernel": f"bp_linear_kernel variant {bp_variant}, {S} slices per launch",
                      "achieved": round(bp_tflops, 2), "peak": FP32_PEAK_TFLOPS,
@@ -507,7 +576,8 @@ def main():
         "bp_f16": bp16,
         "uneven_batch": uneven,
         "differentiable_fbp": adjoint,
-        "streamed_pipeline": streamed,
+        "c1_streamed": streamed,
+        "c0": c0,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

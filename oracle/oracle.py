@@ -86,7 +86,10 @@ def geom(g) -> OrcGeom:
 
 
 def _p(a):
-    return C.c_void_p(a.ctypes.data)
+    # data_as returns a pointer object that keeps `a` alive for as long as the
+    # pointer is: a converted temporary (_p(_c(x, dtype))) must not be freed
+    # before the C function has read it
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def _c(a, dtype):

@@ -894,6 +894,196 @@ __global__ void __launch_bounds__(64 * kAdjWaves) bp_adjoint_kernel(const float*
         }
 }
 
+// ---------------------------------------------------------------------------
+// Sheared-path adjoint projector: the transpose of bp_linear_kernel without
+// per-pixel scatter.
+//
+// For angle a write t(x, y) = (x - c0) cA + (r - c0) sA with c0 = (N-1)/2,
+// where (cA, sA) = scale (cos, sin) and r = y when |cos| >= |sin| (class A,
+// image read as is), else the roles of x and y swap (class B, the image read
+// transposed) -- so |cA| >= |sA| and, for scale <= 1, |cA| <= 1.  Lane j of a
+// wave follows the pixel path x = j + delta(r), delta(r) = -round(rho(r)),
+// rho = (r - c0) sA / cA, one pixel per row r.  On that path
+//     t = base_j + eps(r),  base_j = (j - c0) cA,  eps(r) = cA (rho - round rho),
+// and |eps| <= |cA| / 2, so t stays within one interval of width |cA| <= 1:
+// floor(t) takes at most two values, a_lo = floor(base_j - |cA|/2) and
+// a_lo + 1.  With u = t - a_lo = base' + eps (base' = base_j - a_lo, in
+// [0, 2)) a pixel of value v adds, as the forward's weights (1 - f, f) of its
+// bin floor(t) = a_lo + [u >= 1] dictate,
+//     lo pixel: (1 - u) v to a_lo, u v to a_lo + 1,
+//     hi pixel: (2 - u) v to a_lo + 1, (u - 1) v to a_lo + 2.
+// So a lane only keeps four sums over its path: sum v and sum v eps, over all
+// its pixels and over its hi pixels (hi iff eps >= 1 - base', a per-lane
+// threshold): 6 VALU per pixel-angle (compare, select, two adds, two FMAs)
+// and no inter-lane traffic until the path ends.  base' and eps are formed in
+// double and rounded once, so the weights are the forward's to ~1e-7 (the
+// hat weights are continuous in t, so which side of a bin edge the two
+// roundings put a pixel on changes nothing).
+//
+// The paths of one angle tile every row exactly (bands of 64 lanes, J0 =
+// -max|delta| + 64 b), so every pixel is on exactly one path per angle.
+// Images are read from a zero-padded, masked copy (pixels outside the mask
+// add nothing to the forward's output, so they are zero here) through buffer
+// loads (hardware range check: an out-of-range offset reads 0).  A wave ends
+// its path by adding its lanes' three contributions into a 72-entry LDS
+// window of the bins its band covers and writing that window as a partial;
+// adj_reduce_kernel sums a sinogram row's band partials in band order
+// (deterministic).
+// ---------------------------------------------------------------------------
+constexpr int kAdjWin = 72;                 // bins per band window (64 |cA| + 3 <= 67)
+constexpr int kAdjRec = kAdjWin + 1;        // record: kstart, then the window
+constexpr int kAdjPad = 80;                 // zero columns either side of a padded image row
+constexpr int kAdjWavesPerGroup = 8;        // angles per workgroup (one wave each), one band
+
+struct AdjAngle {
+    int p;          // angle index, -1: padding entry
+    int cls;        // 0: read the image as is, 1: transposed
+    float cA;       // along-lane coefficient (cos or sin), unscaled
+    int J0base;     // -max|delta|
+    int nb;         // bands of this angle
+};
+
+struct AdjParams {
+    int N, P, D, cI;
+    int PW;               // padded row length (N + 2 kAdjPad)
+    int nbmax;            // bands per angle in the partial layout
+    int npos;             // entries of the sorted angle list
+    float scale;
+    double c0;            // (N - 1) / 2
+    float outmul;         // norm * out_scale
+    long img_stride;      // elements per padded image
+};
+
+// Zero-padded, masked copies of `slices` images [N][N]: A [r][kAdjPad + x] =
+// img[r][x], B [r][kAdjPad + x] = img[x][r] (transposed), zero outside the
+// mask.  The padding columns are zeroed once at set-up and never written.
+__global__ void __launch_bounds__(256) adj_pad_kernel(const float* __restrict__ img, float* __restrict__ A,
+                                                      float* __restrict__ B, int N, int PW, int mask, float center,
+                                                      long img_stride)
+{
+    __shared__ float tile[32][33];
+    const int s = blockIdx.z;
+    const float* I = img + (long)s * N * N;
+    float* As = A + s * img_stride;
+    float* Bs = B + s * img_stride;
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int y = by + r, x = bx + threadIdx.x;
+        float v = 0.0f;
+        if (y < N && x < N) {
+            v = I[(long)y * N + x];
+            if (mask && bp_outside(x, y, N, center)) v = 0.0f;
+            As[(long)y * PW + kAdjPad + x] = v;
+        }
+        tile[r][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int x = bx + r, y = by + threadIdx.x;   // B row x, column y
+        if (y < N && x < N) Bs[(long)x * PW + kAdjPad + y] = tile[threadIdx.x][r];
+    }
+}
+
+__global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel(
+    const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ partial,
+    const AdjAngle* __restrict__ angles, const int2* __restrict__ bands, const int2* __restrict__ rowtab, AdjParams q)
+{
+    __shared__ float win[kAdjWavesPerGroup][kAdjWin];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int lane = threadIdx.x % 64;
+    const int pos = blockIdx.x * kAdjWavesPerGroup + wave;
+    const int b = blockIdx.y, sl = blockIdx.z;
+    if (pos >= q.npos) return;
+    const AdjAngle an = angles[pos];
+    if (an.p < 0 || b >= an.nb) return;
+    const int p = an.p;
+    float* rec = partial + (((long)sl * q.P + p) * q.nbmax + b) * kAdjRec;
+    const int J0 = an.J0base + 64 * b;
+    // this lane's path: base_j, its bins a_lo, a_lo + 1 and the hi threshold
+    const double cAd = (double)an.cA * (double)q.scale;
+    const double base = ((double)(J0 + lane) - q.c0) * cAd;
+    const int a_lo = (int)floor(base - 0.5 * fabs(cAd));
+    const float basep = (float)(base - (double)a_lo);
+    const float thr = 1.0f - basep;
+    const int2 rr = bands[p * q.nbmax + b];
+    const float* img = (an.cls ? B : A) + (long)sl * q.img_stride;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(img), (short)0, (int)(q.img_stride * 4), 0x00020000);
+    const int2* rt = rowtab + (long)p * q.N;
+    const int voff = lane * 4;
+    float g0t = 0.0f, g0h = 0.0f, et = 0.0f, eh = 0.0f;
+    int r = rr.x;
+    // eight rows per step: the loads of a step are issued before its sums
+    for (; r + 8 <= rr.y + 1; r += 8) {
+        float v[8], e[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int2 t = rt[r + i];
+            e[i] = __int_as_float(t.y);
+            v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float qv = e[i] >= thr ? v[i] : 0.0f;
+            g0t += v[i];
+            g0h += qv;
+            et = __builtin_fmaf(v[i], e[i], et);
+            eh = __builtin_fmaf(qv, e[i], eh);
+        }
+    }
+    for (; r <= rr.y; r++) {
+        const int2 t = rt[r];
+        const float ev = __int_as_float(t.y);
+        const float vv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
+        const float qv = ev >= thr ? vv : 0.0f;
+        g0t += vv;
+        g0h += qv;
+        et = __builtin_fmaf(vv, ev, et);
+        eh = __builtin_fmaf(qv, ev, eh);
+    }
+    // the path's three contributions (t-bins a_lo, a_lo + 1, a_lo + 2)
+    const float g0l = g0t - g0h, el = et - eh;
+    const float g1l = __builtin_fmaf(basep, g0l, el), g1h = __builtin_fmaf(basep, g0h, eh);
+    const float c0v = g0l - g1l, c1v = g1l + 2.0f * g0h - g1h, c2v = g1h - g0h;
+    // window of the band: the lanes' a_lo are monotone in the lane index
+    const int kstart = min(__builtin_amdgcn_readfirstlane(a_lo), __builtin_amdgcn_readlane(a_lo, 63));
+    float* w = win[wave];
+    for (int i = lane; i < kAdjWin; i += 64) w[i] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int wi = min(max(a_lo - kstart, 0), kAdjWin - 3);
+    atomicAdd(&w[wi], c0v);
+    atomicAdd(&w[wi + 1], c1v);
+    atomicAdd(&w[wi + 2], c2v);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) rec[0] = __int_as_float(kstart);
+    for (int i = lane; i < kAdjWin; i += 64) rec[1 + i] = w[i];
+}
+
+// sino[s][a][k] = outmul * sum over the angle's bands (in band order) of the
+// band windows' entry for t-bin k - cI.
+__global__ void __launch_bounds__(256) adj_reduce_kernel(const float* __restrict__ partial,
+                                                         const int* __restrict__ nb_of, float* __restrict__ sino,
+                                                         AdjParams q)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y, sl = blockIdx.z;
+    if (k >= q.D) return;
+    const float* rec = partial + ((long)sl * q.P + p) * q.nbmax * kAdjRec;
+    const int nb = nb_of[p];
+    const int tb = k - q.cI;
+    float sum = 0.0f;
+    for (int b = 0; b < nb; b++) {
+        const float* rb = rec + (long)b * kAdjRec;
+        const int i = tb - __float_as_int(rb[0]);
+        if (i >= 0 && i < kAdjWin) sum += rb[1 + i];
+    }
+    sino[((long)sl * q.P + p) * q.D + k] = sum * q.outmul;
+}
+
 // Standalone Masking (Masking.cu:113-144): optional min-max normalisation
 // (two-pass reduction), then the circular mask.
 __global__ void minmax_partial_kernel(const float* __restrict__ img, long n, float2* __restrict__ partial)
@@ -1016,8 +1206,23 @@ struct risa_bp_s {
     int variant_f16 = 0;
     bool fuse_mask = false;
     risa::DeviceBuffer<uint2> x4;   // staging for risa_hip_bp_run_f16's plain layout [kF16StageGroups][P][D]
-    risa::DeviceBuffer<float> imgT;  // transposed image for risa_hip_bp_adjoint_run [N][N]
+    risa::DeviceBuffer<float> imgT;  // transposed image for the fallback adjoint [N][N]
+    // sheared-path adjoint (scale <= 1): tables and per-pass buffers
+    bool adj_path = false;
+    risa::AdjParams adj{};
+    int adj_groups = 0;                      // workgroups along the sorted angle list
+    risa::DeviceBuffer<risa::AdjAngle> adj_angles;
+    risa::DeviceBuffer<int2> adj_rows;       // [P][N] {r * PW + pad + delta(r), bits of eps(r)}
+    risa::DeviceBuffer<int2> adj_bands;      // [P][nbmax] {first row, last row}
+    risa::DeviceBuffer<int> adj_nb;          // [P]
+    risa::DeviceBuffer<float> adj_imgA, adj_imgB;   // [kAdjSlices][N][PW] padded, masked copies
+    risa::DeviceBuffer<float> adj_partial;   // [kAdjSlices][P][nbmax][kAdjRec]
 };
+
+namespace risa {
+// slices per pass of the sheared-path adjoint through its buffers
+constexpr int kAdjSlices = 2;
+}  // namespace risa
 
 namespace risa {
 // risa_hip_bp_run_f16 converts and backprojects up to this many groups of 4
@@ -1069,6 +1274,104 @@ static BpParams make_params(const risa_bp_s* h, int tx, int ty)
     return p;
 }
 
+// Tables of the sheared-path adjoint (bp_adjoint_path_kernel) in double on
+// the host: per angle its class, path shifts delta(r), residuals eps(r) and
+// bands; per band the rows whose path segment meets the mask.  Applies when
+// scale * max(|cos|, |sin|) <= 1 for every angle (a lane's t then stays
+// within two bins); otherwise the handle keeps the gather adjoint.
+static void setup_adjoint(risa_bp_s* h)
+{
+    const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
+    const double scale = (double)h->lut.scale;
+    for (int a = 0; a < P; a++) {
+        const double c = h->lut.cosT[a], sn = h->lut.sinT[a];
+        if (std::max(std::fabs(c), std::fabs(sn)) * scale > 1.0) return;
+    }
+    const int PW = N + 2 * kAdjPad;
+    const double c0 = (N - 1) * 0.5;
+    const auto inside = mask_intervals(N, h->fuse_mask);
+    std::vector<AdjAngle> cls[2];
+    std::vector<int2> rows((size_t)P * N);
+    std::vector<int> nb(P), dm(P);
+    for (int a = 0; a < P; a++) {
+        const double c = h->lut.cosT[a], sn = h->lut.sinT[a];
+        const int k = std::fabs(c) >= std::fabs(sn) ? 0 : 1;
+        const double cA = k ? sn : c, sA = k ? c : sn;
+        int m = 0;
+        for (int r = 0; r < N; r++) {
+            const double rho = (r - c0) * (sA / cA);
+            const double rd = std::floor(rho + 0.5);
+            const int delta = -(int)rd;
+            const float eps = (float)(cA * scale * (rho - rd));
+            rows[(size_t)a * N + r] = make_int2(r * PW + kAdjPad + delta, __builtin_bit_cast(int, eps));
+            m = std::max(m, std::abs(delta));
+        }
+        dm[a] = m;
+        nb[a] = ceil_div(N + 2 * m, 64);
+        cls[k].push_back(AdjAngle{a, k, (float)cA, -m, nb[a]});
+    }
+    int nbmax = 0;
+    for (int a = 0; a < P; a++) nbmax = std::max(nbmax, nb[a]);
+    // rows of each band whose path segment meets the mask (one pixel of
+    // margin); every row in between must stay inside the padded row
+    std::vector<int2> bands((size_t)P * nbmax, make_int2(1, 0));
+    for (int a = 0; a < P; a++)
+        for (int b = 0; b < nb[a]; b++) {
+            const int J0 = -dm[a] + 64 * b;
+            int first = -1, last = -2;
+            for (int r = 0; r < N; r++) {
+                const int lo = inside[(size_t)r].first - 1, hi = inside[(size_t)r].second + 1;
+                if (inside[(size_t)r].first > inside[(size_t)r].second) continue;
+                const int x0 = rows[(size_t)a * N + r].x - r * PW - kAdjPad + J0;
+                if (x0 + 63 < lo || x0 > hi) continue;
+                if (first < 0) first = r;
+                last = r;
+            }
+            if (first < 0) continue;
+            for (int r = first; r <= last; r++) {
+                const int x0 = rows[(size_t)a * N + r].x - r * PW - kAdjPad + J0;
+                if (x0 < -kAdjPad || x0 + 63 >= N + kAdjPad) return;   // would leave the padded row: keep the gather adjoint
+            }
+            bands[(size_t)a * nbmax + b] = make_int2(first, last);
+        }
+    // sorted angle list: class A, then class B, each padded to whole workgroups
+    std::vector<AdjAngle> list;
+    for (int k = 0; k < 2; k++) {
+        list.insert(list.end(), cls[k].begin(), cls[k].end());
+        while (list.size() % kAdjWavesPerGroup) list.push_back(AdjAngle{-1, 0, 0.0f, 0, 0});
+    }
+    AdjParams q{};
+    q.N = N;
+    q.P = P;
+    q.D = D;
+    q.cI = (int)(D * 0.5);
+    q.PW = PW;
+    q.nbmax = nbmax;
+    q.npos = (int)list.size();
+    q.scale = h->lut.scale;
+    q.c0 = c0;
+    q.outmul = h->lut.norm;
+    q.img_stride = (long)N * PW;
+    h->adj = q;
+    h->adj_groups = (int)list.size() / kAdjWavesPerGroup;
+    h->adj_angles.resize(list.size());
+    h->adj_angles.upload(list);
+    h->adj_rows.resize(rows.size());
+    h->adj_rows.upload(rows);
+    h->adj_bands.resize(bands.size());
+    h->adj_bands.upload(bands);
+    h->adj_nb.resize(nb.size());
+    h->adj_nb.upload(nb);
+    h->adj_imgA.resize((size_t)kAdjSlices * q.img_stride);
+    h->adj_imgB.resize((size_t)kAdjSlices * q.img_stride);
+    // the padding columns stay zero: pad kernels write only the N x N interior
+    RISA_HIP_CHECK(hipMemset(h->adj_imgA.ptr, 0, h->adj_imgA.count * sizeof(float)));
+    RISA_HIP_CHECK(hipMemset(h->adj_imgB.ptr, 0, h->adj_imgB.count * sizeof(float)));
+    h->adj_partial.resize((size_t)kAdjSlices * P * nbmax * kAdjRec);
+    RISA_HIP_CHECK(hipDeviceSynchronize());
+    h->adj_path = true;
+}
+
 extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out)
 {
     return guarded("risa_hip_bp_create", [&] {
@@ -1088,7 +1391,8 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
             // run-call scratch, sized here so run calls allocate nothing
             const size_t plane = (size_t)g->numberOfParallelProjections * g->numberOfParallelDetectors;
             h->x4.resize((size_t)kF16StageGroups * plane);
-            h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);
+            setup_adjoint(h);
+            if (!h->adj_path) h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);
         } catch (...) {
             delete h;
             throw;
@@ -1188,33 +1492,74 @@ extern "C" int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_s
     });
 }
 
+// The gather adjoint (bp_adjoint_kernel) of one slice: geometries the
+// sheared-path kernel does not cover (scale > 1).
+static void adjoint_gather(risa_bp_t h, const float* img, float* sino, hipStream_t s)
+{
+    // the transpose of the FP32 linear kernel the handle runs: same tile shape
+    const BpVariant& v = kVariants[h->variant];
+    BpParams p = make_params(h, v.tx, v.ty);
+    const int tilesY = ceil_div(p.N, v.ty);
+    int ltx = 0, lty = 0;
+    while ((1 << ltx) < v.tx) ltx++;
+    while ((1 << lty) < v.ty) lty++;
+    if ((1 << ltx) != v.tx || (1 << lty) != v.ty) fail(RISA_ERR_UNSUPPORTED, "adjoint needs power-of-two tile edges");
+    // a line's span for kAdjK detectors: (kAdjK + 2) / (|cos or sin| scale)
+    // pixels with |cos or sin| >= 1/sqrt(2), plus slack and rounding
+    const int cap = std::min(p.N, (int)std::ceil((kAdjK + 2) * 1.41422 / p.scale) + 6);
+    const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + kAdjWaves * kAdjK * sizeof(float) +
+                       (size_t)kAdjWaves * cap * 12;
+    if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
+    hipLaunchKernelGGL(transpose_kernel, dim3(ceil_div(p.N, 32), ceil_div(p.N, 32)), dim3(32, 8), 0, s, img,
+                       h->imgT.ptr, p.N);
+    check_launch("transpose_kernel");
+    dim3 grid(ceil_div(p.D, kAdjK), p.P);
+    hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64 * kAdjWaves), lds, s, img, h->imgT.ptr, sino, h->cosT.ptr,
+                       h->sinT.ptr, p, ltx, lty, tilesY, cap);
+    check_launch("bp_adjoint_kernel");
+}
+
+// `slices` images [N][N] -> sinograms [P][D]: passes of up to kAdjSlices
+// slices through the handle's padded copies and partial windows (allocated
+// at create; stream order keeps a pass behind the previous one).
+static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, hipStream_t s)
+{
+    const int N = h->g.numberOfPixels;
+    const long P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
+    if (!h->adj_path) {
+        for (int i = 0; i < slices; i++) adjoint_gather(h, img + (size_t)i * N * N, sino + (size_t)i * P * D, s);
+        return;
+    }
+    AdjParams q = h->adj;
+    for (int s0 = 0; s0 < slices; s0 += kAdjSlices) {
+        const int n = std::min(kAdjSlices, slices - s0);
+        hipLaunchKernelGGL(adj_pad_kernel, dim3(ceil_div(N, 32), ceil_div(N, 32), n), dim3(32, 8), 0, s,
+                           img + (size_t)s0 * N * N, h->adj_imgA.ptr, h->adj_imgB.ptr, N, q.PW, h->fuse_mask ? 1 : 0,
+                           h->lut.center, q.img_stride);
+        check_launch("adj_pad_kernel");
+        hipLaunchKernelGGL(bp_adjoint_path_kernel, dim3(h->adj_groups, q.nbmax, n), dim3(64 * kAdjWavesPerGroup), 0,
+                           s, h->adj_imgA.ptr, h->adj_imgB.ptr, h->adj_partial.ptr, h->adj_angles.ptr,
+                           h->adj_bands.ptr, h->adj_rows.ptr, q);
+        check_launch("bp_adjoint_path_kernel");
+        hipLaunchKernelGGL(adj_reduce_kernel, dim3(ceil_div(D, 256), P, n), dim3(256), 0, s, h->adj_partial.ptr,
+                           h->adj_nb.ptr, sino + (size_t)s0 * P * D, q);
+        check_launch("adj_reduce_kernel");
+    }
+}
+
 extern "C" int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream)
 {
     return guarded("risa_hip_bp_adjoint_run", [&] {
         if (!h || !img || !sino) fail(RISA_ERR_ARG, "null argument");
-        // the transpose of the FP32 linear kernel the handle runs: same tile shape
-        const BpVariant& v = kVariants[h->variant];
-        BpParams p = make_params(h, v.tx, v.ty);
-        const int tilesY = ceil_div(p.N, v.ty);
-        int ltx = 0, lty = 0;
-        while ((1 << ltx) < v.tx) ltx++;
-        while ((1 << lty) < v.ty) lty++;
-        if ((1 << ltx) != v.tx || (1 << lty) != v.ty)
-            fail(RISA_ERR_UNSUPPORTED, "adjoint needs power-of-two tile edges");
-        // a line's span for kAdjK detectors: (kAdjK + 2) / (|cos or sin| scale)
-        // pixels with |cos or sin| >= 1/sqrt(2), plus slack and rounding
-        const int cap = std::min(p.N, (int)std::ceil((kAdjK + 2) * 1.41422 / p.scale) + 6);
-        const size_t lds = (size_t)p.tilesX * tilesY * 8 + (size_t)p.N * 8 + kAdjWaves * kAdjK * sizeof(float) +
-                           (size_t)kAdjWaves * cap * 12;
-        if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "adjoint: its tables need %zu B of LDS", lds);
-        hipStream_t s = as_stream(stream);
-        hipLaunchKernelGGL(transpose_kernel, dim3(ceil_div(p.N, 32), ceil_div(p.N, 32)), dim3(32, 8), 0, s, img,
-                           h->imgT.ptr, p.N);
-        check_launch("transpose_kernel");
-        dim3 grid(ceil_div(p.D, kAdjK), p.P);
-        hipLaunchKernelGGL(bp_adjoint_kernel, grid, dim3(64 * kAdjWaves), lds, s, img, h->imgT.ptr, sino, h->cosT.ptr,
-                           h->sinT.ptr, p, ltx, lty, tilesY, cap);
-        check_launch("bp_adjoint_kernel");
+        adjoint_run(h, img, 1, sino, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_bp_adjoint_run_batch(risa_bp_t h, const float* img, int slices, float* sino, void* stream)
+{
+    return guarded("risa_hip_bp_adjoint_run_batch", [&] {
+        if (!h || !img || !sino || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (slices) adjoint_run(h, img, slices, sino, as_stream(stream));
     });
 }
 

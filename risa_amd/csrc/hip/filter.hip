@@ -1173,23 +1173,50 @@ extern "C" int risa_hip_filter_adjoint_run(risa_filter_t h, float* grad, int sli
     });
 }
 
+// Where a tap-gradient call keeps its partial sums: the handle's buffers, or
+// the allocation-free entry point's static array.
+struct GradScratch {
+    float* partial;    // M == D: [kTapBlocks][D/2 + 1]
+    float2* xpart;     // M != D: [kTapBlocks][M/2 + 1]
+    double2* xsum;     // [M/2 + 1]
+    double* xcorr;     // [D]
+};
+
 static void launch_grad_taps_padded(risa_filter_t h, const float* x, const float* g, long rows, int nblocks,
-                                    float* dtaps, hipStream_t s)
+                                    float* dtaps, const GradScratch& sc, hipStream_t s)
 {
     const int D = h->g.numberOfParallelDetectors, M = h->M, KM = M / 2 + 1;
     const int slots = fft_slots(M);
     const size_t lds = (size_t)slots * 2 * M * sizeof(float2);
     const int grid = (int)std::max(1L, std::min<long>(nblocks, (rows + slots - 1) / slots));
     hipLaunchKernelGGL(fft_xspec_kernel, dim3(grid), dim3(128 * slots), lds, s, x, g, h->tw.ptr, h->plan, D, rows,
-                       h->xpart.ptr);
+                       sc.xpart);
     check_launch("fft_xspec_kernel");
-    hipLaunchKernelGGL(xspec_reduce_kernel, dim3(ceil_div(KM, 256)), dim3(256), 0, s, h->xpart.ptr, grid, KM,
-                       h->xsum.ptr);
+    hipLaunchKernelGGL(xspec_reduce_kernel, dim3(ceil_div(KM, 256)), dim3(256), 0, s, sc.xpart, grid, KM, sc.xsum);
     check_launch("xspec_reduce_kernel");
-    hipLaunchKernelGGL(xcorr_wrap_kernel, dim3(ceil_div(D, 64)), dim3(64), 0, s, h->xsum.ptr, M, D, h->xcorr.ptr);
+    hipLaunchKernelGGL(xcorr_wrap_kernel, dim3(ceil_div(D, 64)), dim3(64), 0, s, sc.xsum, M, D, sc.xcorr);
     check_launch("xcorr_wrap_kernel");
-    hipLaunchKernelGGL(xcorr_taps_kernel, dim3(ceil_div(D / 2 + 1, 64)), dim3(64), 0, s, h->xcorr.ptr, D, dtaps);
+    hipLaunchKernelGGL(xcorr_taps_kernel, dim3(ceil_div(D / 2 + 1, 64)), dim3(64), 0, s, sc.xcorr, D, dtaps);
     check_launch("xcorr_taps_kernel");
+}
+
+static void grad_taps(risa_filter_t h, const float* x, const float* g, int slices, float* dtaps, const GradScratch& sc,
+                      hipStream_t s)
+{
+    const int D = h->g.numberOfParallelDetectors, K = D / 2 + 1, M = h->M;
+    const long rows = (long)slices * h->g.numberOfParallelProjections;
+    if (M != D) {
+        launch_grad_taps_padded(h, x, g, rows, kTapBlocks, dtaps, sc, s);
+        return;
+    }
+    const int slots = fft_slots(D);
+    const int nblocks = (int)std::max(1L, std::min<long>(kTapBlocks, (rows + slots - 1) / slots));
+    const size_t lds = (size_t)slots * 2 * D * sizeof(float2);
+    hipLaunchKernelGGL(fft_grad_taps_kernel, dim3(nblocks), dim3(128 * slots), lds, s, x, g, h->tw.ptr, h->plan, rows,
+                       sc.partial);
+    check_launch("fft_grad_taps_kernel");
+    hipLaunchKernelGGL(reduce_taps_kernel, dim3(K), dim3(256), 0, s, sc.partial, nblocks, K, D, dtaps);
+    check_launch("reduce_taps_kernel");
 }
 
 extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const float* g, int slices, float* dtaps,
@@ -1197,21 +1224,8 @@ extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const 
 {
     return guarded("risa_hip_filter_grad_taps", [&] {
         if (!h || !x || !g || !dtaps || slices < 0) fail(RISA_ERR_ARG, "bad argument");
-        const int D = h->g.numberOfParallelDetectors, K = D / 2 + 1, M = h->M;
-        const long rows = (long)slices * h->g.numberOfParallelProjections;
-        hipStream_t s = as_stream(stream);
-        if (M != D) {
-            launch_grad_taps_padded(h, x, g, rows, kTapBlocks, dtaps, s);
-            return;
-        }
-        const int slots = fft_slots(D);
-        const int nblocks = (int)std::max(1L, std::min<long>(kTapBlocks, (rows + slots - 1) / slots));
-        const size_t lds = (size_t)slots * 2 * D * sizeof(float2);
-        hipLaunchKernelGGL(fft_grad_taps_kernel, dim3(nblocks), dim3(128 * slots), lds, s, x, g, h->tw.ptr, h->plan,
-                           rows, h->scratch.ptr);
-        check_launch("fft_grad_taps_kernel");
-        hipLaunchKernelGGL(reduce_taps_kernel, dim3(K), dim3(256), 0, s, h->scratch.ptr, nblocks, K, D, dtaps);
-        check_launch("reduce_taps_kernel");
+        const GradScratch sc{h->scratch.ptr, h->xpart.ptr, h->xsum.ptr, h->xcorr.ptr};
+        grad_taps(h, x, g, slices, dtaps, sc, as_stream(stream));
     });
 }
 

@@ -408,6 +408,28 @@ std::vector<float> filter_taps(const risa_geom& g)
     return h;
 }
 
+std::vector<std::pair<int, int>> mask_intervals(int N, bool mask)
+{
+    std::vector<std::pair<int, int>> out((size_t)N, {0, N - 1});
+    if (!mask) return out;
+    const float center = (float)((N - 1.0) * 0.5);
+    auto outside = [&](int x, int y) {
+        const float dX = (float)x - center, dY = (float)y - center;
+        const float dist = dX * dX + dY * dY;   // rounded products and sum (no contraction here)
+        return (double)dist > (double)N * N * 0.25;
+    };
+    for (int y = 0; y < N; y++) {
+        int lo = N, hi = -1;
+        for (int x = 0; x < N; x++)
+            if (!outside(x, y)) {
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+            }
+        out[(size_t)y] = {lo, hi};
+    }
+    return out;
+}
+
 BpLut bp_lut(const risa_geom& g)
 {
     // Backprojection::processor, Backprojection.cu:118-141.

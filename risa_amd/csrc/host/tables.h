@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "risa_hip.h"
@@ -60,6 +61,13 @@ PackedRebin pack_rebin(const risa_geom& g, const RebinTables& t, bool goal_angle
 
 // Filter taps h[0..D/2] (Filter.cu:169-191).
 std::vector<float> filter_taps(const risa_geom& g);
+
+// Per image row y, the columns inside the reconstruction mask as one
+// interval {lo, hi} (lo > hi: none): the Masking test (Masking.cu:157-169) --
+// float squared distance from the centre (N-1)/2 with rounded products,
+// compared in double with N*N/4 -- which is symmetric in x and y and
+// monotone in |x - centre|.  Without the mask every row is {0, N-1}.
+std::vector<std::pair<int, int>> mask_intervals(int N, bool mask);
 
 // Backprojection LUT (Backprojection.cu:118-141).
 struct BpLut {

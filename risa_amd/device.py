@@ -231,6 +231,11 @@ class Backprojection(_Handle):
         check(load().risa_hip_bp_adjoint_run(self.h, ptr(img), ptr(sino), _stream()))
         return sino
 
+    def adjoint_batch(self, img, slices: int, sino):
+        _require_cuda(img, sino)
+        check(load().risa_hip_bp_adjoint_run_batch(self.h, ptr(img), slices, ptr(sino), _stream()))
+        return sino
+
     def autotune(self):
         check(load().risa_hip_bp_autotune(self.h, _stream()))
 

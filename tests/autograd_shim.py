@@ -1,7 +1,7 @@
 """torch.autograd.Function over the C-ABI (ctypes), for the test harness only.
 
 Filtered backprojection  img = BP(F_h(sino))  with the hand-written adjoint:
-  d sino = F_h^T BP^T d img      (ray-driven projector, then the conjugate-window filter)
+  d sino = F_h^T BP^T d img      (the sheared-path projector, then the conjugate-window filter)
   d h    = grad_taps(sino, BP^T d img)
 Only the unfiltered input is kept for the backward pass; the filtered sinogram
 and the image are recomputed-free (the operators are linear), so nothing of
@@ -21,8 +21,7 @@ class FilteredBackprojection(torch.autograd.Function):
         x = sino.detach().contiguous().clone()
         filt(x, S)
         img = torch.empty((S, N, N), dtype=torch.float32, device=sino.device)
-        for s in range(S):
-            bp(x[s], img[s])
+        bp.batch(x, S, img)
         ctx.save_for_backward(sino.detach())
         ctx.filt, ctx.bp, ctx.taps_device = filt, bp, taps.device
         return img
@@ -34,8 +33,7 @@ class FilteredBackprojection(torch.autograd.Function):
         S, P, D = sino.shape
         gimg = gimg.contiguous()
         gs = torch.empty((S, P, D), dtype=torch.float32, device=sino.device)
-        for s in range(S):
-            bp.adjoint(gimg[s], gs[s])
+        bp.adjoint_batch(gimg, S, gs)
         dtaps = torch.empty(D // 2 + 1, dtype=torch.float32, device=sino.device)
         filt.grad_taps(sino.contiguous(), gs, S, dtaps)
         filt.adjoint(gs, S)
