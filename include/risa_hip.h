@@ -268,6 +268,16 @@ int risa_hip_filter_adjoint_run(risa_filter_t h, float* grad, int slices, void* 
  * dtaps: device [D/2+1], overwritten. */
 int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const float* g, int slices,
                               float* dtaps, void* stream);
+/* risa_hip_filter_grad_taps with its partial sums in a static device array
+ * instead of the handle's buffers: the call allocates nothing (for callers
+ * that cannot allocate: code under stream capture, tight serving loops).
+ * No counterpart in the reference.  The array (8.4 MB per GPU, held until
+ * the process ends) fits any filter a handle accepts (D <= 2048); larger
+ * needs are refused with RISA_ERR_UNSUPPORTED.  All calls on one GPU share
+ * it: calls that overlap on one GPU (different streams, no ordering between
+ * them) race and give wrong results. */
+int risa_hip_filter_grad_taps_noalloc(risa_filter_t h, const float* x, const float* g, int slices,
+                                      float* dtaps, void* stream);
 int risa_hip_filter_destroy(risa_filter_t h);
 
 /* --------------------------------------------------------- Backprojection -- */

@@ -173,6 +173,7 @@ _SIGS = {
     "risa_hip_filter_run_f16x4": [P, P, P, C.c_int, C.c_float, P],
     "risa_hip_filter_adjoint_run": [P, P, C.c_int, P],
     "risa_hip_filter_grad_taps": [P, P, P, C.c_int, P, P],
+    "risa_hip_filter_grad_taps_noalloc": [P, P, P, C.c_int, P, P],
     "risa_hip_filter_destroy": [P],
     "risa_hip_bp_create": [C.POINTER(Geom), C.c_int, C.POINTER(P)],
     "risa_hip_bp_run": [P, P, P, P],

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "../host/tables.h"
 
@@ -981,6 +983,8 @@ static int fft_length(int D, FftPlan& plan)
     return 0;
 }
 
+static void resolve_scratch(int device);
+
 extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, risa_filter_t* out)
 {
     return guarded("risa_hip_filter_create", [&] {
@@ -1025,6 +1029,7 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
             }
             h->taps = filter_taps(*g);
             upload_response(h);
+            resolve_scratch(device);
         } catch (...) {
             delete h;
             throw;
@@ -1225,6 +1230,63 @@ extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const 
     return guarded("risa_hip_filter_grad_taps", [&] {
         if (!h || !x || !g || !dtaps || slices < 0) fail(RISA_ERR_ARG, "bad argument");
         const GradScratch sc{h->scratch.ptr, h->xpart.ptr, h->xsum.ptr, h->xcorr.ptr};
+        grad_taps(h, x, g, slices, dtaps, sc, as_stream(stream));
+    });
+}
+
+// ------------------------------------------- allocation-free tap gradient --
+// The partial sums of risa_hip_filter_grad_taps_noalloc live in `scratch`, a
+// file-scope device array (one per GPU, held until the process ends) sized
+// for the largest filter a handle accepts (D <= 2048; transform length
+// M <= 4096): kTapBlocks x (M/2 + 1) complex partials, the summed spectrum
+// and the wrapped cross-correlation -- 8.4 MB.  Host code never passes the
+// array's name to a kernel: each device's address comes from
+// hipGetSymbolAddress, resolved once per device when a filter handle is
+// created there (set-up, outside any capture), and kept in scratch_ptr.
+constexpr size_t kScratchPartialBytes = (size_t)kTapBlocks * (kMaxFftLen / 2 + 1) * sizeof(float2);
+constexpr size_t kScratchSumBytes = (size_t)(kMaxFftLen / 2 + 1) * sizeof(double2);
+constexpr size_t kScratchCorrBytes = (size_t)kMaxDetectors * sizeof(double);
+constexpr size_t kScratchBytes = kScratchPartialBytes + kScratchSumBytes + kScratchCorrBytes;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex scratch_m;
+static std::map<int, unsigned char*> scratch_ptr;   // device -> address of that device's `scratch`
+
+// With `device` current: looks up (once) that device's copy of `scratch`.
+static void resolve_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(scratch_m);
+    if (scratch_ptr.count(device)) return;
+    void* ptr = nullptr;
+    RISA_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) fail(RISA_ERR_HIP, "hipGetSymbolAddress returned no address for the tap-gradient scratch");
+    scratch_ptr[device] = reinterpret_cast<unsigned char*>(ptr);
+}
+
+extern "C" int risa_hip_filter_grad_taps_noalloc(risa_filter_t h, const float* x, const float* g, int slices,
+                                                 float* dtaps, void* stream)
+{
+    return guarded("risa_hip_filter_grad_taps_noalloc", [&] {
+        if (!h || !x || !g || !dtaps || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        unsigned char* base = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(scratch_m);
+            auto it = scratch_ptr.find(h->device);
+            if (it == scratch_ptr.end()) fail(RISA_ERR_ARG, "no static scratch resolved for device %d", h->device);
+            base = it->second;
+        }
+        const int D = h->g.numberOfParallelDetectors, M = h->M;
+        const size_t need = M == D ? (size_t)kTapBlocks * (D / 2 + 1) * sizeof(float)
+                                   : (size_t)kTapBlocks * (M / 2 + 1) * sizeof(float2) + (size_t)(M / 2 + 1) *
+                                     sizeof(double2) + (size_t)D * sizeof(double);
+        if (D > kMaxDetectors || M > kMaxFftLen || need > kScratchBytes)
+            fail(RISA_ERR_UNSUPPORTED, "tap gradient of %d detectors needs %zu B of scratch; the static array holds %zu B",
+                 D, need, kScratchBytes);
+        GradScratch sc{};
+        sc.partial = reinterpret_cast<float*>(base);
+        sc.xpart = reinterpret_cast<float2*>(base);
+        sc.xsum = reinterpret_cast<double2*>(base + kScratchPartialBytes);
+        sc.xcorr = reinterpret_cast<double*>(base + kScratchPartialBytes + kScratchSumBytes);
         grad_taps(h, x, g, slices, dtaps, sc, as_stream(stream));
     });
 }
