@@ -895,30 +895,36 @@ __global__ void __launch_bounds__(64 * kAdjWaves) bp_adjoint_kernel(const float*
 }
 
 // ---------------------------------------------------------------------------
-// Sheared-path adjoint projector: the transpose of bp_linear_kernel without
-// per-pixel scatter.
+// Sheared-path adjoint projector: the exact transpose of bp_linear_kernel
+// (the variant in use) without per-pixel scatter.
 //
 // For angle a write t(x, y) = (x - c0) cA + (r - c0) sA with c0 = (N-1)/2,
 // where (cA, sA) = scale (cos, sin) and r = y when |cos| >= |sin| (class A,
 // image read as is), else the roles of x and y swap (class B, the image read
 // transposed) -- so |cA| >= |sA| and, for scale <= 1, |cA| <= 1.  Lane j of a
 // wave follows the pixel path x = j + delta(r), delta(r) = -round(rho(r)),
-// rho = (r - c0) sA / cA, one pixel per row r.  On that path
-//     t = base_j + eps(r),  base_j = (j - c0) cA,  eps(r) = cA (rho - round rho),
-// and |eps| <= |cA| / 2, so t stays within one interval of width |cA| <= 1:
-// floor(t) takes at most two values, a_lo = floor(base_j - |cA|/2) and
-// a_lo + 1.  With u = t - a_lo = base' + eps (base' = base_j - a_lo, in
-// [0, 2)) a pixel of value v adds, as the forward's weights (1 - f, f) of its
-// bin floor(t) = a_lo + [u >= 1] dictate,
-//     lo pixel: (1 - u) v to a_lo, u v to a_lo + 1,
-//     hi pixel: (2 - u) v to a_lo + 1, (u - 1) v to a_lo + 2.
-// So a lane only keeps four sums over its path: sum v and sum v eps, over all
-// its pixels and over its hi pixels (hi iff eps >= 1 - base', a per-lane
-// threshold): 6 VALU per pixel-angle (compare, select, two adds, two FMAs)
-// and no inter-lane traffic until the path ends.  base' and eps are formed in
-// double and rounded once, so the weights are the forward's to ~1e-7 (the
-// hat weights are continuous in t, so which side of a bin edge the two
-// roundings put a pixel on changes nothing).
+// rho = (r - c0) sA / cA, one pixel per row r.  On that path the exact t is
+// base_j + cA (rho - round rho), within |cA| / 2 <= 1/2 of base_j =
+// (j - c0) cA, so its bin floor(t) takes at most two values, a_lo =
+// floor(base_j - |cA|/2) and a_lo + 1, and the lane feeds at most the three
+// bins a_lo .. a_lo + 2.
+//
+// The weights are the forward's own: per pixel the lane recomputes the
+// forward's float coordinate t' = fma(yp, sin, fma(xp, cos, -T)) with T the
+// integer origin of the pixel's forward tile (same operations, same tile
+// table), so the forward's bin is a_lo + floor(u) with u = t' + T - a_lo
+// (exact: an integer shift of a small float) and its fraction u - floor(u).
+// With m = u - 1 in [-1, 1) a pixel of value v adds max(-m, 0) v to a_lo,
+// (1 - |m|) v to a_lo + 1 and max(m, 0) v to a_lo + 2, so a lane keeps three
+// sums over its path: sum v, sum v m and sum v |m| (|m| is a free source
+// modifier of the FMA).  Only the pixel
+// coordinate of the lane's axis and T vary along a path inside one forward
+// tile row (class A) or column (class B); the wave tabulates them in LDS per
+// such segment (class A: fma(xp, cos, -T) and T per pixel column; class B:
+// yp and -T per pixel row), so a row costs one LDS read and 7 (class A) or
+// 8 (class B) VALU operations per lane.  A pixel whose forward coordinate
+// rounds across the lane's outer bin edges (|m| > 1 by ~1e-5) keeps weights
+// off by that much: a set of measure ~1e-5 of the pixels.
 //
 // The paths of one angle tile every row exactly (bands of 64 lanes, J0 =
 // -max|delta| + 64 b), so every pixel is on exactly one path per angle.
@@ -934,6 +940,7 @@ constexpr int kAdjWin = 72;                 // bins per band window (64 |cA| + 3
 constexpr int kAdjRec = kAdjWin + 1;        // record: kstart, then the window
 constexpr int kAdjPad = 80;                 // zero columns either side of a padded image row
 constexpr int kAdjWavesPerGroup = 8;        // angles per workgroup (one wave each), one band
+constexpr int kAdjTab = 192;                // segment table entries per wave (64 + 64 + 1 used at most)
 
 struct AdjAngle {
     int p;          // angle index, -1: padding entry
@@ -948,11 +955,32 @@ struct AdjParams {
     int PW;               // padded row length (N + 2 kAdjPad)
     int nbmax;            // bands per angle in the partial layout
     int npos;             // entries of the sorted angle list
-    float scale;
+    float scale, center;  // the forward's pixel coordinate ((float)x - center) * scale
     double c0;            // (N - 1) / 2
     float outmul;         // norm * out_scale
     long img_stride;      // elements per padded image
+    int lgTX, lgTY;       // log2 of the forward tile's width / height
+    int tilesX, tilesY;   // the forward's tile grid
 };
+
+// The forward's tile origins T[a][tile row][tile column] for tiles of
+// TX x TY (bp_tile's angle_window on the same inputs) and its row
+// coordinates ((float)r - center) * scale.
+__global__ void adj_tile_origin_kernel(const float* __restrict__ cosT, const float* __restrict__ sinT, AdjParams q,
+                                       float* __restrict__ T, float* __restrict__ coord)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < q.N) coord[i] = ((float)i - q.center) * q.scale;
+    const long tiles = (long)q.tilesX * q.tilesY;
+    if (i >= tiles * q.P) return;
+    const int a = (int)(i / tiles);
+    const int trow = (int)(i % tiles) / q.tilesX, tcol = (int)(i % tiles) % q.tilesX;
+    const int x0 = tcol << q.lgTX, y0 = trow << q.lgTY;
+    const float xa = ((float)x0 - q.center) * q.scale, xb = ((float)(x0 + (1 << q.lgTX) - 1) - q.center) * q.scale;
+    const float ya = ((float)y0 - q.center) * q.scale, yb = ((float)(y0 + (1 << q.lgTY) - 1) - q.center) * q.scale;
+    const AngleWin w = angle_window(cosT[a], sinT[a], xa, xb, ya, yb, q.cI);
+    T[i] = -w.negT;
+}
 
 // Zero-padded, masked copies of `slices` images [N][N]: A [r][kAdjPad + x] =
 // img[r][x], B [r][kAdjPad + x] = img[x][r] (transposed), zero outside the
@@ -984,11 +1012,104 @@ __global__ void __launch_bounds__(256) adj_pad_kernel(const float* __restrict__ 
     }
 }
 
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One lane's path over rows [r0, r1] of angle p (class CLS); returns its
+// three sums.  tab: this wave's segment table.
+template <int CLS>
+__device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, const int2* __restrict__ rt,
+                                         const float* __restrict__ coord, const float* __restrict__ Tp,
+                                         const AdjParams& q, float2* tab, int J0, int lane, int r0, int r1, float c,
+                                         float s, float A1, float& g0, float& gm, float& ga)
+{
+    // the segment: rows of one forward tile row (class A) or column (class B)
+    const int sh = CLS ? q.lgTX : q.lgTY;
+    const int voff = lane * 4;
+    const unsigned lbase = lds_address(tab) + (unsigned)lane * 8u;
+    int r = r0, dmin = 0;
+    // the row's table offset stays scalar: one VALU add per LDS address
+    // (left to itself the compiler re-associates it into two)
+    auto entry = [&](int d) {
+        unsigned a;
+        asm("v_add_u32 %0, %1, %2" : "=v"(a) : "s"((d - dmin) * 8), "v"(lbase));
+        const lds_f2 w = lds_load<lds_f2>(a);
+        return make_float2(w.x, w.y);
+    };
+    while (r <= r1) {
+        if (r == r0 || (r & ((1 << sh) - 1)) == 0) {
+            const int re = min(r1, (((r >> sh) + 1) << sh) - 1);
+            const int da = rt[r].y, db = rt[re].y;   // delta is monotone in r
+            dmin = min(da, db);
+            const int n = 64 + abs(db - da);
+            wave_lds_sync();
+            for (int i = lane; i < n; i += 64) {
+                const int pos = J0 + dmin + i;
+                const float pc = ((float)pos - q.center) * q.scale;
+                if (CLS == 0) {
+                    const int tcol = min(max(pos >> q.lgTX, 0), q.tilesX - 1);
+                    const float T = Tp[(r >> q.lgTY) * q.tilesX + tcol];
+                    tab[i] = make_float2(__builtin_fmaf(pc, c, -T), T);
+                } else {
+                    const int trow = min(max(pos >> q.lgTY, 0), q.tilesY - 1);
+                    const float T = Tp[trow * q.tilesX + (r >> q.lgTX)];
+                    tab[i] = make_float2(pc, -T);
+                }
+            }
+            wave_lds_sync();
+        }
+        auto row = [&](float v, float2 e, float rc) {
+            float tp, rm;
+            if (CLS == 0) {
+                tp = __builtin_fmaf(rc, s, e.x);                       // fma(yp, sin, xc)
+                rm = e.y - A1;
+            } else {
+                tp = __builtin_fmaf(e.x, s, __builtin_fmaf(rc, c, e.y));   // fma(yp, sin, fma(xp, cos, -T))
+                rm = -e.y - A1;
+            }
+            const float m = tp + rm;
+            g0 += v;
+            gm = __builtin_fmaf(v, m, gm);
+            ga = __builtin_fmaf(v, fabsf(m), ga);   // |m|: a source modifier, no extra operation
+        };
+        const int nrow = min(8 - (r & 7), r1 - r + 1);
+        if (nrow == 8) {
+            // eight rows per step: the loads of a step are issued before its sums
+            float v[8], rc[8];
+            float2 e[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int2 t = rt[r + i];
+                v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
+                e[i] = entry(t.y);
+                rc[i] = coord[r + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) row(v[i], e[i], rc[i]);
+        } else {
+            for (int i = 0; i < nrow; i++) {
+                const int2 t = rt[r + i];
+                const float v =
+                    __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
+                row(v, entry(t.y), coord[r + i]);
+            }
+        }
+        r += nrow;
+    }
+}
+
 __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel(
     const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ partial,
-    const AdjAngle* __restrict__ angles, const int2* __restrict__ bands, const int2* __restrict__ rowtab, AdjParams q)
+    const AdjAngle* __restrict__ angles, const int2* __restrict__ bands, const int2* __restrict__ rowtab,
+    const float* __restrict__ coord, const float* __restrict__ Ttab, const float* __restrict__ cosT,
+    const float* __restrict__ sinT, AdjParams q)
 {
     __shared__ float win[kAdjWavesPerGroup][kAdjWin];
+    __shared__ float2 tabs[kAdjWavesPerGroup][kAdjTab];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int lane = threadIdx.x % 64;
     const int pos = blockIdx.x * kAdjWavesPerGroup + wave;
@@ -999,66 +1120,35 @@ __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel
     const int p = an.p;
     float* rec = partial + (((long)sl * q.P + p) * q.nbmax + b) * kAdjRec;
     const int J0 = an.J0base + 64 * b;
-    // this lane's path: base_j, its bins a_lo, a_lo + 1 and the hi threshold
+    // this lane's bins a_lo .. a_lo + 2
     const double cAd = (double)an.cA * (double)q.scale;
     const double base = ((double)(J0 + lane) - q.c0) * cAd;
     const int a_lo = (int)floor(base - 0.5 * fabs(cAd));
-    const float basep = (float)(base - (double)a_lo);
-    const float thr = 1.0f - basep;
     const int2 rr = bands[p * q.nbmax + b];
     const float* img = (an.cls ? B : A) + (long)sl * q.img_stride;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(img), (short)0, (int)(q.img_stride * 4), 0x00020000);
     const int2* rt = rowtab + (long)p * q.N;
-    const int voff = lane * 4;
-    float g0t = 0.0f, g0h = 0.0f, et = 0.0f, eh = 0.0f;
-    int r = rr.x;
-    // eight rows per step: the loads of a step are issued before its sums
-    for (; r + 8 <= rr.y + 1; r += 8) {
-        float v[8], e[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int2 t = rt[r + i];
-            e[i] = __int_as_float(t.y);
-            v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const float qv = e[i] >= thr ? v[i] : 0.0f;
-            g0t += v[i];
-            g0h += qv;
-            et = __builtin_fmaf(v[i], e[i], et);
-            eh = __builtin_fmaf(qv, e[i], eh);
-        }
-    }
-    for (; r <= rr.y; r++) {
-        const int2 t = rt[r];
-        const float ev = __int_as_float(t.y);
-        const float vv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
-        const float qv = ev >= thr ? vv : 0.0f;
-        g0t += vv;
-        g0h += qv;
-        et = __builtin_fmaf(vv, ev, et);
-        eh = __builtin_fmaf(qv, ev, eh);
-    }
-    // the path's three contributions (t-bins a_lo, a_lo + 1, a_lo + 2)
-    const float g0l = g0t - g0h, el = et - eh;
-    const float g1l = __builtin_fmaf(basep, g0l, el), g1h = __builtin_fmaf(basep, g0h, eh);
-    const float c0v = g0l - g1l, c1v = g1l + 2.0f * g0h - g1h, c2v = g1h - g0h;
+    const float* Tp = Ttab + (long)p * q.tilesX * q.tilesY;
+    const float c = cosT[p], s = sinT[p], A1 = (float)(a_lo + 1);
+    float g0 = 0.0f, gm = 0.0f, ga = 0.0f;
+    if (an.cls)
+        adj_path<1>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
+    else
+        adj_path<0>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
+    // the path's three contributions (t-bins a_lo, a_lo + 1, a_lo + 2):
+    // max(-m, 0) = (|m| - m) / 2, 1 - |m|, max(m, 0) = (|m| + m) / 2
+    const float c0v = 0.5f * (ga - gm), c1v = g0 - ga, c2v = 0.5f * (ga + gm);
     // window of the band: the lanes' a_lo are monotone in the lane index
     const int kstart = min(__builtin_amdgcn_readfirstlane(a_lo), __builtin_amdgcn_readlane(a_lo, 63));
     float* w = win[wave];
     for (int i = lane; i < kAdjWin; i += 64) w[i] = 0.0f;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
     const int wi = min(max(a_lo - kstart, 0), kAdjWin - 3);
     atomicAdd(&w[wi], c0v);
     atomicAdd(&w[wi + 1], c1v);
     atomicAdd(&w[wi + 2], c2v);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
     if (lane == 0) rec[0] = __int_as_float(kstart);
     for (int i = lane; i < kAdjWin; i += 64) rec[1 + i] = w[i];
 }
@@ -1212,11 +1302,13 @@ struct risa_bp_s {
     risa::AdjParams adj{};
     int adj_groups = 0;                      // workgroups along the sorted angle list
     risa::DeviceBuffer<risa::AdjAngle> adj_angles;
-    risa::DeviceBuffer<int2> adj_rows;       // [P][N] {r * PW + pad + delta(r), bits of eps(r)}
+    risa::DeviceBuffer<int2> adj_rows;       // [P][N] {r * PW + pad + delta(r), delta(r)}
     risa::DeviceBuffer<int2> adj_bands;      // [P][nbmax] {first row, last row}
     risa::DeviceBuffer<int> adj_nb;          // [P]
     risa::DeviceBuffer<float> adj_imgA, adj_imgB;   // [kAdjSlices][N][PW] padded, masked copies
     risa::DeviceBuffer<float> adj_partial;   // [kAdjSlices][P][nbmax][kAdjRec]
+    risa::DeviceBuffer<float> adj_T;         // [P][tilesY][tilesX] the forward's tile origins (variant in use)
+    risa::DeviceBuffer<float> adj_coord;     // [N] the forward's pixel coordinates
 };
 
 namespace risa {
@@ -1275,10 +1367,11 @@ static BpParams make_params(const risa_bp_s* h, int tx, int ty)
 }
 
 // Tables of the sheared-path adjoint (bp_adjoint_path_kernel) in double on
-// the host: per angle its class, path shifts delta(r), residuals eps(r) and
-// bands; per band the rows whose path segment meets the mask.  Applies when
-// scale * max(|cos|, |sin|) <= 1 for every angle (a lane's t then stays
-// within two bins); otherwise the handle keeps the gather adjoint.
+// the host: per angle its class, path shifts delta(r) and bands; per band the
+// rows whose path segment meets the mask.  Applies when scale * max(|cos|,
+// |sin|) <= 1 for every angle (a lane's t then stays within two bins);
+// otherwise the handle keeps the gather adjoint.  The forward's tile origins
+// depend on the variant in use and are rebuilt on the device per call.
 static void setup_adjoint(risa_bp_s* h)
 {
     const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
@@ -1302,8 +1395,7 @@ static void setup_adjoint(risa_bp_s* h)
             const double rho = (r - c0) * (sA / cA);
             const double rd = std::floor(rho + 0.5);
             const int delta = -(int)rd;
-            const float eps = (float)(cA * scale * (rho - rd));
-            rows[(size_t)a * N + r] = make_int2(r * PW + kAdjPad + delta, __builtin_bit_cast(int, eps));
+            rows[(size_t)a * N + r] = make_int2(r * PW + kAdjPad + delta, delta);
             m = std::max(m, std::abs(delta));
         }
         dm[a] = m;
@@ -1349,6 +1441,7 @@ static void setup_adjoint(risa_bp_s* h)
     q.nbmax = nbmax;
     q.npos = (int)list.size();
     q.scale = h->lut.scale;
+    q.center = h->lut.center;
     q.c0 = c0;
     q.outmul = h->lut.norm;
     q.img_stride = (long)N * PW;
@@ -1368,6 +1461,11 @@ static void setup_adjoint(risa_bp_s* h)
     RISA_HIP_CHECK(hipMemset(h->adj_imgA.ptr, 0, h->adj_imgA.count * sizeof(float)));
     RISA_HIP_CHECK(hipMemset(h->adj_imgB.ptr, 0, h->adj_imgB.count * sizeof(float)));
     h->adj_partial.resize((size_t)kAdjSlices * P * nbmax * kAdjRec);
+    size_t tiles = 0;
+    for (int v = 0; v < kNumVariants; v++)
+        tiles = std::max(tiles, (size_t)ceil_div(N, kVariants[v].tx) * ceil_div(N, kVariants[v].ty));
+    h->adj_T.resize(tiles * P);
+    h->adj_coord.resize(N);
     RISA_HIP_CHECK(hipDeviceSynchronize());
     h->adj_path = true;
 }
@@ -1522,6 +1620,15 @@ static void adjoint_gather(risa_bp_t h, const float* img, float* sino, hipStream
 // `slices` images [N][N] -> sinograms [P][D]: passes of up to kAdjSlices
 // slices through the handle's padded copies and partial windows (allocated
 // at create; stream order keeps a pass behind the previous one).
+// log2 of a power-of-two tile edge
+static int ilog2(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) l++;
+    if ((1 << l) != v) fail(RISA_ERR_UNSUPPORTED, "tile edge %d is not a power of two", v);
+    return l;
+}
+
 static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, hipStream_t s)
 {
     const int N = h->g.numberOfPixels;
@@ -1531,6 +1638,16 @@ static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, 
         return;
     }
     AdjParams q = h->adj;
+    // the forward's tile grid of the variant in use, and its tile origins
+    const BpVariant& v = kVariants[h->variant];
+    q.lgTX = ilog2(v.tx);
+    q.lgTY = ilog2(v.ty);
+    q.tilesX = ceil_div(N, v.tx);
+    q.tilesY = ceil_div(N, v.ty);
+    const long work = std::max((long)q.tilesX * q.tilesY * P, (long)N);
+    hipLaunchKernelGGL(adj_tile_origin_kernel, dim3((unsigned)ceil_div(work, 256L)), dim3(256), 0, s, h->cosT.ptr,
+                       h->sinT.ptr, q, h->adj_T.ptr, h->adj_coord.ptr);
+    check_launch("adj_tile_origin_kernel");
     for (int s0 = 0; s0 < slices; s0 += kAdjSlices) {
         const int n = std::min(kAdjSlices, slices - s0);
         hipLaunchKernelGGL(adj_pad_kernel, dim3(ceil_div(N, 32), ceil_div(N, 32), n), dim3(32, 8), 0, s,
@@ -1539,7 +1656,8 @@ static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, 
         check_launch("adj_pad_kernel");
         hipLaunchKernelGGL(bp_adjoint_path_kernel, dim3(h->adj_groups, q.nbmax, n), dim3(64 * kAdjWavesPerGroup), 0,
                            s, h->adj_imgA.ptr, h->adj_imgB.ptr, h->adj_partial.ptr, h->adj_angles.ptr,
-                           h->adj_bands.ptr, h->adj_rows.ptr, q);
+                           h->adj_bands.ptr, h->adj_rows.ptr, h->adj_coord.ptr, h->adj_T.ptr, h->cosT.ptr, h->sinT.ptr,
+                           q);
         check_launch("bp_adjoint_path_kernel");
         hipLaunchKernelGGL(adj_reduce_kernel, dim3(ceil_div(D, 256), P, n), dim3(256), 0, s, h->adj_partial.ptr,
                            h->adj_nb.ptr, sino + (size_t)s0 * P * D, q);

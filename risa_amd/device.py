@@ -188,11 +188,25 @@ class Filter(_Handle):
         return t
 
 
-def bp_kernel_name(f16: bool, variant: int) -> str:
-    """The backprojection kernel a variant launches, as profilers name it."""
+def bp_num_variants(f16: bool = False) -> int:
+    """Number of FP32 (or FP16-input) backprojection tile variants."""
+    shape = (C.c_int * 7)()
+    n = 0
+    while load().risa_hip_bp_variant_shape(1 if f16 else 0, n, shape) == 0:
+        n += 1
+    return n
+
+
+def bp_variant_shape(f16: bool, variant: int) -> tuple:
+    """(tile width, tile height, pixels per thread, E, U, G, max angles) of a variant."""
     shape = (C.c_int * 7)()
     check(load().risa_hip_bp_variant_shape(1 if f16 else 0, variant, shape))
-    tx, ty, ppt, e, u, g, _ = list(shape)
+    return tuple(shape)
+
+
+def bp_kernel_name(f16: bool, variant: int) -> str:
+    """The backprojection kernel a variant launches, as profilers name it."""
+    tx, ty, ppt, e, u, g, _ = bp_variant_shape(f16, variant)
     if f16:
         return f"risa::bp_linear_f16_kernel<{tx}, {ty}, {ppt}, {e}, {u}, {g}>"
     return f"risa::bp_linear_kernel<{tx}, {ty}, {ppt}, {e}, {u}>"

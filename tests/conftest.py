@@ -92,6 +92,18 @@ def rel_rmse(a, ref, mask=None):
     return float(np.sqrt(np.mean((a - ref) ** 2)) / (den if den > 0 else 1.0))
 
 
+def dot_gap(pairs):
+    """Relative gap of the dot-product test <A u, v> = <u, A^T v> summed over
+    random pairs: sum |lhs - rhs| / sum |lhs|.  One pair's ratio is
+    ill-conditioned -- with independent zero-mean vectors <A u, v> lands near
+    0 now and then, and float32 summation noise (about 1e-6 of the inner
+    product's typical size) then reads as a large ratio; summing over pairs
+    keeps the denominator at its typical size.  pairs: [(lhs, rhs), ...]."""
+    num = sum(abs(a - b) for a, b in pairs)
+    den = sum(abs(a) for a, _ in pairs)
+    return num / den
+
+
 def disk_mask(N):
     c = (N - 1) * 0.5
     y, x = np.mgrid[0:N, 0:N].astype(np.float32)

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import disk_mask, rel_rmse
+from conftest import disk_mask, dot_gap, rel_rmse
 
 pytestmark = pytest.mark.gpu
 
@@ -193,7 +193,7 @@ def test_adjoint_vs_fp64_oracle_batch_and_dot(risa, orc, geom_c0, cuda, N, P, D,
     backprojector on masked images (rel RMSE <= 2e-5: float angle table
     against the oracle's exact angles), the batched entry bit-equal to
     per-slice calls, and the dot-product test <BP s, x> = <s, BP^T x> against
-    the GPU's own forward (<= 1e-5)."""
+    the GPU's own forward (relative gap over 8 random pairs <= 1e-5)."""
     import torch
     from risa_amd import device
 
@@ -218,15 +218,164 @@ def test_adjoint_vs_fp64_oracle_batch_and_dot(risa, orc, geom_c0, cuda, N, P, D,
         err = rel_rmse(batch[i].cpu().numpy(), ref)
         print(f"  slice {i}: rel RMSE vs FP64 transpose {err:.2e}", flush=True)
         assert err < 2e-5
-    s = rng.standard_normal((P, D)).astype(np.float32)
-    bs = torch.empty((N, N), dtype=torch.float32, device=cuda)
-    bp(_t(s, cuda), bs)
-    _sync()
-    lhs = float(np.sum(np.where(m, bs.cpu().numpy().astype(np.float64), 0.0) * x[0]))
-    rhs = float(np.sum(s.astype(np.float64) * batch[0].cpu().numpy()))
-    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
-    print(f"  dot test rel gap {gap:.2e}", flush=True)
+    pairs = []
+    for trial in range(8):
+        s = rng.standard_normal((P, D)).astype(np.float32)
+        xi = rng.standard_normal((N, N)).astype(np.float32)
+        bs = torch.empty((N, N), dtype=torch.float32, device=cuda)
+        btx = torch.empty((P, D), dtype=torch.float32, device=cuda)
+        bp(_t(s, cuda), bs)
+        bp.adjoint(_t(xi, cuda), btx)
+        _sync()
+        pairs.append((float(np.sum(np.where(m, bs.cpu().numpy().astype(np.float64), 0.0) * xi)),
+                      float(np.sum(s.astype(np.float64) * btx.cpu().numpy()))))
+    gap = dot_gap(pairs)
+    print(f"  dot test over 8 pairs: rel gap {gap:.2e} (" +
+          ", ".join(f"{a:.6e}/{b:.6e}" for a, b in pairs) + ")", flush=True)
     assert gap <= 1e-5
+    bp.close()
+
+
+def _lut(g):
+    from risa_amd import _lib
+
+    P = g.numberOfParallelProjections
+    s, c, k = np.empty(P, np.float32), np.empty(P, np.float32), np.empty(3, np.float32)
+    _lib.check(_lib.load().risa_bp_lut(g, _lib.ptr(s), _lib.ptr(c), _lib.ptr(k)))
+    return s, c, k
+
+
+_WEIGHT_CASES = [(128, 90, 128, -30.0, 0), (97, 41, 80, 13.0, 1), (64, 36, 64, 0.0, 1)]
+
+
+@pytest.mark.parametrize("N,P,D,rot,mask", _WEIGHT_CASES)
+def test_forward_weights_match_rounding_model(risa, geom_c0, cuda, N, P, D, rot, mask):
+    """The FP32 linear backprojector's weights, read off one-hot sinograms
+    (one sample per slice, so each output pixel is one weight times norm),
+    equal the float32 model of its coordinates bit for bit (to one ulp of
+    the product with norm), for every tile variant."""
+    import torch
+    import bp_rounding as br
+    from risa_amd import device
+
+    print(f"case: forward weights N={N} P={P} D={D} rot={rot} mask={mask}", flush=True)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=P, numberOfParallelDetectors=D,
+                        rotationOffset=rot, fusedMasking=mask, filterType=0, maskingValue=0.0)
+    sT, cT, k3 = _lut(g)
+    scale, norm, center = k3
+    cI = int(D * 0.5)
+    y, x = np.mgrid[0:N, 0:N]
+    inside = ~br.outside(x, y, N, center) if mask else np.ones((N, N), bool)
+    rng = np.random.default_rng(N + P)
+    S = 24
+    angles = rng.choice(P, S, replace=False)
+    bins = rng.integers(D // 4, 3 * D // 4, S)
+    sino = np.zeros((S, P, D), np.float32)
+    sino[np.arange(S), angles, bins] = 1.0
+    bp = device.Backprojection(g)
+    worst = 0.0
+    for v in range(device.bp_num_variants()):
+        TX, TY = device.bp_variant_shape(False, v)[:2]
+        bp.variant = v
+        img = torch.empty((S, N, N), dtype=torch.float32, device=cuda)
+        bp.batch(_t(sino, cuda), S, img)
+        _sync()
+        got = img.cpu().numpy()
+        for i in range(S):
+            a, kk = angles[i], bins[i]
+            k, f = br.weights(x, y, cT[a], sT[a], TX, TY, center, scale, cI)
+            w = np.where(k == kk, (1.0 - f).astype(np.float32), np.where(k + 1 == kk, f.astype(np.float32), 0.0))
+            want = (w.astype(np.float32) * norm).astype(np.float32)
+            err = np.abs(np.where(inside, got[i] - want, 0.0)).max()
+            worst = max(worst, err / norm)
+            assert err <= np.spacing(norm), f"variant {v} ({TX}x{TY}) angle {a} bin {kk}: max |diff| {err:.3e}"
+    print(f"  worst |GPU - model| over 10 variants x {S} angles: {worst:.2e} of norm", flush=True)
+    bp.close()
+
+
+@pytest.mark.parametrize("N,P,D,rot,mask", _WEIGHT_CASES)
+def test_adjoint_weights_match_rounding_model(risa, geom_c0, cuda, N, P, D, rot, mask):
+    """The adjoint projector's weights, read off one-hot images (one pixel per
+    slice, so each output sample is one weight times norm), equal the same
+    float32 model of the forward's coordinates (a few ulps of norm), for
+    every tile variant: the adjoint is the exact transpose of the forward
+    variant in use."""
+    import torch
+    import bp_rounding as br
+    from risa_amd import device
+
+    print(f"case: adjoint weights N={N} P={P} D={D} rot={rot} mask={mask}", flush=True)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=P, numberOfParallelDetectors=D,
+                        rotationOffset=rot, fusedMasking=mask, filterType=0, maskingValue=0.0)
+    sT, cT, k3 = _lut(g)
+    scale, norm, center = k3
+    cI = int(D * 0.5)
+    rng = np.random.default_rng(3 * N + P)
+    S = 16
+    px = rng.integers(0, N, 4 * S)
+    py = rng.integers(0, N, 4 * S)
+    keep = ~br.outside(px, py, N, center) if mask else np.ones(px.shape, bool)
+    px, py = px[keep][:S], py[keep][:S]
+    assert len(px) == S
+    img = np.zeros((S, N, N), np.float32)
+    img[np.arange(S), py, px] = 1.0
+    bp = device.Backprojection(g)
+    worst = 0.0
+    for v in range(device.bp_num_variants()):
+        TX, TY = device.bp_variant_shape(False, v)[:2]
+        bp.variant = v
+        sino = torch.empty((S, P, D), dtype=torch.float32, device=cuda)
+        bp.adjoint_batch(_t(img, cuda), S, sino)
+        _sync()
+        got = sino.cpu().numpy()
+        for i in range(S):
+            want = np.zeros((P, D))
+            k, f = br.weights(px[i], py[i], cT, sT, TX, TY, center, scale, cI)
+            for kk, ww in ((k, 1.0 - f), (k + 1, f)):
+                ok = (kk >= 0) & (kk < D)
+                want[np.arange(P)[ok], kk[ok]] += ww[ok] * float(norm)
+            err = np.abs(got[i] - want).max()
+            worst = max(worst, err / norm)
+            assert err <= 4 * np.spacing(norm), \
+                f"variant {v} ({TX}x{TY}) pixel ({px[i]}, {py[i]}): max |diff| {err:.3e}"
+    print(f"  worst |GPU - model| over 10 variants x {S} pixels: {worst:.2e} of norm", flush=True)
+    bp.close()
+
+
+@pytest.mark.parametrize("N,P,D,rot,mask", [(257, 181, 200, 13.0, 1), (128, 90, 128, -30.0, 0)])
+def test_adjoint_dot_every_tile_variant(risa, orc, geom_c0, cuda, N, P, D, rot, mask):
+    """The adjoint is the transpose of the forward tile variant in use (its
+    weights recompute the forward's float coordinates, tile origins
+    included): <BP s, x> = <s, BP^T x>, relative gap over 8 random pairs
+    <= 1e-5 for every FP32 variant."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: adjoint dot, every variant, N={N} P={P} D={D} rot={rot} mask={mask}", flush=True)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=P, numberOfParallelDetectors=D,
+                        rotationOffset=rot, fusedMasking=mask, filterType=0)
+    m = disk_mask(N) if mask else np.ones((N, N), bool)
+    bp = device.Backprojection(g)
+    rng = np.random.default_rng(7 * N + P)
+    worst = 0.0
+    for v in range(device.bp_num_variants()):
+        bp.variant = v
+        pairs = []
+        for trial in range(8):
+            s = rng.standard_normal((P, D)).astype(np.float32)
+            x = rng.standard_normal((N, N)).astype(np.float32)
+            bs = torch.empty((N, N), dtype=torch.float32, device=cuda)
+            btx = torch.empty((P, D), dtype=torch.float32, device=cuda)
+            bp(_t(s, cuda), bs)
+            bp.adjoint(_t(x, cuda), btx)
+            _sync()
+            pairs.append((float(np.sum(np.where(m, bs.cpu().numpy().astype(np.float64), 0.0) * x)),
+                          float(np.sum(s.astype(np.float64) * btx.cpu().numpy()))))
+        gap = dot_gap(pairs)
+        worst = max(worst, gap)
+        print(f"  variant {v}: dot gap over 8 pairs {gap:.2e}", flush=True)
+        assert gap <= 1e-5, f"variant {v}: " + ", ".join(f"{a:.9e}/{b:.9e}" for a, b in pairs)
+    print(f"  worst dot gap over {device.bp_num_variants()} variants: {worst:.2e}", flush=True)
     bp.close()
 
 
