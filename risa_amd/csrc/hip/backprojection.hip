@@ -20,6 +20,7 @@
 #include "device_common.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #include <cmath>
 
@@ -1020,7 +1021,10 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // One lane's path over rows [r0, r1] of angle p (class CLS); returns its
-// three sums.  tab: this wave's segment table.
+// three sums.  tab: this wave's segment table.  rt[r] = {byte offset of the
+// path's row-r pixel of lane 0 in the padded image, minus J0 * 4; delta(r) * 8}.
+// Rows go in blocks of 16 (8 where a segment has fewer rows left; a partial
+// block's spare rows add zero), loads first.
 template <int CLS>
 __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, const int2* __restrict__ rt,
                                          const float* __restrict__ coord, const float* __restrict__ Tp,
@@ -1029,22 +1033,59 @@ __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, cons
 {
     // the segment: rows of one forward tile row (class A) or column (class B)
     const int sh = CLS ? q.lgTX : q.lgTY;
-    const int voff = lane * 4;
-    const unsigned lbase = lds_address(tab) + (unsigned)lane * 8u;
-    int r = r0, dmin = 0;
-    // the row's table offset stays scalar: one VALU add per LDS address
-    // (left to itself the compiler re-associates it into two)
-    auto entry = [&](int d) {
+    const int voff = lane * 4, J04 = J0 * 4;
+    const unsigned lane8 = lds_address(tab) + (unsigned)lane * 8u;
+    unsigned lbase = lane8;   // lane8 - 8 * (the segment's smallest delta)
+    // the row's table entry: one VALU add on a scalar byte offset (left to
+    // itself the compiler re-associates the address into two)
+    auto entry = [&](int d8) {
         unsigned a;
-        asm("v_add_u32 %0, %1, %2" : "=v"(a) : "s"((d - dmin) * 8), "v"(lbase));
+        asm("v_add_u32 %0, %1, %2" : "=v"(a) : "s"(d8), "v"(lbase));
         const lds_f2 w = lds_load<lds_f2>(a);
         return make_float2(w.x, w.y);
     };
+    auto load = [&](int off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, off + J04, 0));
+    };
+    auto row = [&](float v, float2 e, float rc) {
+        float tp, rm;
+        if (CLS == 0) {
+            tp = __builtin_fmaf(rc, s, e.x);                           // fma(yp, sin, xc)
+            rm = e.y - A1;
+        } else {
+            tp = __builtin_fmaf(e.x, s, __builtin_fmaf(rc, c, e.y));   // fma(yp, sin, fma(xp, cos, -T))
+            rm = -e.y - A1;
+        }
+        const float m = tp + rm;
+        g0 += v;
+        gm = __builtin_fmaf(v, m, gm);
+        ga = __builtin_fmaf(v, fabsf(m), ga);   // |m|: a source modifier, no extra operation
+    };
+    // NB rows from r, of which the first `live` count (the rest re-read row
+    // r + live - 1, in bounds, and add zero)
+    auto block = [&](auto nb, int r, int live) {
+        constexpr int NB = decltype(nb)::value;
+        float v[NB], rc[NB];
+        float2 e[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int ri = r + min(i, live - 1);
+            const int2 t = rt[ri];
+            v[i] = load(t.x);
+            if (i >= live) v[i] = 0.0f;
+            e[i] = entry(t.y);
+            rc[i] = coord[ri];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) row(v[i], e[i], rc[i]);
+    };
+    int r = r0;
     while (r <= r1) {
+        const int segend = (((r >> sh) + 1) << sh) - 1;
         if (r == r0 || (r & ((1 << sh) - 1)) == 0) {
-            const int re = min(r1, (((r >> sh) + 1) << sh) - 1);
-            const int da = rt[r].y, db = rt[re].y;   // delta is monotone in r
-            dmin = min(da, db);
+            const int re = min(r1, segend);
+            const int da = rt[r].y >> 3, db = rt[re].y >> 3;   // delta is monotone in r
+            const int dmin = min(da, db);
             const int n = 64 + abs(db - da);
             wave_lds_sync();
             for (int i = lane; i < n; i += 64) {
@@ -1060,45 +1101,18 @@ __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, cons
                     tab[i] = make_float2(pc, -T);
                 }
             }
+            lbase = lane8 - (unsigned)(dmin * 8);
             wave_lds_sync();
         }
-        auto row = [&](float v, float2 e, float rc) {
-            float tp, rm;
-            if (CLS == 0) {
-                tp = __builtin_fmaf(rc, s, e.x);                       // fma(yp, sin, xc)
-                rm = e.y - A1;
-            } else {
-                tp = __builtin_fmaf(e.x, s, __builtin_fmaf(rc, c, e.y));   // fma(yp, sin, fma(xp, cos, -T))
-                rm = -e.y - A1;
-            }
-            const float m = tp + rm;
-            g0 += v;
-            gm = __builtin_fmaf(v, m, gm);
-            ga = __builtin_fmaf(v, fabsf(m), ga);   // |m|: a source modifier, no extra operation
-        };
-        const int nrow = min(8 - (r & 7), r1 - r + 1);
-        if (nrow == 8) {
-            // eight rows per step: the loads of a step are issued before its sums
-            float v[8], rc[8];
-            float2 e[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int2 t = rt[r + i];
-                v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
-                e[i] = entry(t.y);
-                rc[i] = coord[r + i];
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++) row(v[i], e[i], rc[i]);
+        // rows left in this segment (segments are whole tile rows / columns)
+        const int lim = min(r1, segend) - r + 1;
+        if (lim >= 16) {
+            block(std::integral_constant<int, 16>{}, r, 16);
+            r += 16;
         } else {
-            for (int i = 0; i < nrow; i++) {
-                const int2 t = rt[r + i];
-                const float v =
-                    __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, (t.x + J0) * 4, 0));
-                row(v, entry(t.y), coord[r + i]);
-            }
+            block(std::integral_constant<int, 8>{}, r, min(lim, 8));
+            r += min(lim, 8);
         }
-        r += nrow;
     }
 }
 
@@ -1113,10 +1127,14 @@ __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int lane = threadIdx.x % 64;
     const int pos = blockIdx.x * kAdjWavesPerGroup + wave;
-    const int b = blockIdx.y, sl = blockIdx.z;
+    const int sl = blockIdx.z;
     if (pos >= q.npos) return;
     const AdjAngle an = angles[pos];
-    if (an.p < 0 || b >= an.nb) return;
+    if (an.p < 0 || (int)blockIdx.y >= an.nb) return;
+    // bands centre-out: the long paths through the middle of the image are
+    // dispatched first, the short edge bands fill in behind them
+    const int i = blockIdx.y, mid = (an.nb - 1) / 2;
+    const int b = (i & 1) ? mid + (i + 1) / 2 : mid - i / 2;
     const int p = an.p;
     float* rec = partial + (((long)sl * q.P + p) * q.nbmax + b) * kAdjRec;
     const int J0 = an.J0base + 64 * b;
@@ -1154,24 +1172,28 @@ __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel
 }
 
 // sino[s][a][k] = outmul * sum over the angle's bands (in band order) of the
-// band windows' entry for t-bin k - cI.
+// band windows' entry for t-bin k - cI.  One workgroup per sinogram row: the
+// row's band records are staged in LDS with coalesced loads first.
 __global__ void __launch_bounds__(256) adj_reduce_kernel(const float* __restrict__ partial,
                                                          const int* __restrict__ nb_of, float* __restrict__ sino,
                                                          AdjParams q)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    const int p = blockIdx.y, sl = blockIdx.z;
-    if (k >= q.D) return;
-    const float* rec = partial + ((long)sl * q.P + p) * q.nbmax * kAdjRec;
+    extern __shared__ float recs[];
+    const int p = blockIdx.x, sl = blockIdx.y;
     const int nb = nb_of[p];
-    const int tb = k - q.cI;
-    float sum = 0.0f;
-    for (int b = 0; b < nb; b++) {
-        const float* rb = rec + (long)b * kAdjRec;
-        const int i = tb - __float_as_int(rb[0]);
-        if (i >= 0 && i < kAdjWin) sum += rb[1 + i];
+    const float* src = partial + ((long)sl * q.P + p) * q.nbmax * kAdjRec;
+    for (int i = threadIdx.x; i < nb * kAdjRec; i += 256) recs[i] = src[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < q.D; k += 256) {
+        const int tb = k - q.cI;
+        float sum = 0.0f;
+        for (int b = 0; b < nb; b++) {
+            const float* rb = recs + b * kAdjRec;
+            const int i = tb - __float_as_int(rb[0]);
+            if (i >= 0 && i < kAdjWin) sum += rb[1 + i];
+        }
+        sino[((long)sl * q.P + p) * q.D + k] = sum * q.outmul;
     }
-    sino[((long)sl * q.P + p) * q.D + k] = sum * q.outmul;
 }
 
 // Standalone Masking (Masking.cu:113-144): optional min-max normalisation
@@ -1302,7 +1324,7 @@ struct risa_bp_s {
     risa::AdjParams adj{};
     int adj_groups = 0;                      // workgroups along the sorted angle list
     risa::DeviceBuffer<risa::AdjAngle> adj_angles;
-    risa::DeviceBuffer<int2> adj_rows;       // [P][N] {r * PW + pad + delta(r), delta(r)}
+    risa::DeviceBuffer<int2> adj_rows;       // [P][N] {(r * PW + pad + delta(r)) * 4, delta(r) * 8}
     risa::DeviceBuffer<int2> adj_bands;      // [P][nbmax] {first row, last row}
     risa::DeviceBuffer<int> adj_nb;          // [P]
     risa::DeviceBuffer<float> adj_imgA, adj_imgB;   // [kAdjSlices][N][PW] padded, masked copies
@@ -1313,7 +1335,7 @@ struct risa_bp_s {
 
 namespace risa {
 // slices per pass of the sheared-path adjoint through its buffers
-constexpr int kAdjSlices = 2;
+constexpr int kAdjSlices = 8;
 }  // namespace risa
 
 namespace risa {
@@ -1395,7 +1417,7 @@ static void setup_adjoint(risa_bp_s* h)
             const double rho = (r - c0) * (sA / cA);
             const double rd = std::floor(rho + 0.5);
             const int delta = -(int)rd;
-            rows[(size_t)a * N + r] = make_int2(r * PW + kAdjPad + delta, delta);
+            rows[(size_t)a * N + r] = make_int2((r * PW + kAdjPad + delta) * 4, delta * 8);
             m = std::max(m, std::abs(delta));
         }
         dm[a] = m;
@@ -1404,6 +1426,7 @@ static void setup_adjoint(risa_bp_s* h)
     }
     int nbmax = 0;
     for (int a = 0; a < P; a++) nbmax = std::max(nbmax, nb[a]);
+    if ((size_t)nbmax * kAdjRec * sizeof(float) > kMaxLds) return;   // a row's band records must fit the reduce's LDS
     // rows of each band whose path segment meets the mask (one pixel of
     // margin); every row in between must stay inside the padded row
     std::vector<int2> bands((size_t)P * nbmax, make_int2(1, 0));
@@ -1414,14 +1437,14 @@ static void setup_adjoint(risa_bp_s* h)
             for (int r = 0; r < N; r++) {
                 const int lo = inside[(size_t)r].first - 1, hi = inside[(size_t)r].second + 1;
                 if (inside[(size_t)r].first > inside[(size_t)r].second) continue;
-                const int x0 = rows[(size_t)a * N + r].x - r * PW - kAdjPad + J0;
+                const int x0 = rows[(size_t)a * N + r].y / 8 + J0;
                 if (x0 + 63 < lo || x0 > hi) continue;
                 if (first < 0) first = r;
                 last = r;
             }
             if (first < 0) continue;
             for (int r = first; r <= last; r++) {
-                const int x0 = rows[(size_t)a * N + r].x - r * PW - kAdjPad + J0;
+                const int x0 = rows[(size_t)a * N + r].y / 8 + J0;
                 if (x0 < -kAdjPad || x0 + 63 >= N + kAdjPad) return;   // would leave the padded row: keep the gather adjoint
             }
             bands[(size_t)a * nbmax + b] = make_int2(first, last);
@@ -1659,8 +1682,8 @@ static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, 
                            h->adj_bands.ptr, h->adj_rows.ptr, h->adj_coord.ptr, h->adj_T.ptr, h->cosT.ptr, h->sinT.ptr,
                            q);
         check_launch("bp_adjoint_path_kernel");
-        hipLaunchKernelGGL(adj_reduce_kernel, dim3(ceil_div(D, 256), P, n), dim3(256), 0, s, h->adj_partial.ptr,
-                           h->adj_nb.ptr, sino + (size_t)s0 * P * D, q);
+        hipLaunchKernelGGL(adj_reduce_kernel, dim3(P, n), dim3(256), (size_t)q.nbmax * kAdjRec * sizeof(float), s,
+                           h->adj_partial.ptr, h->adj_nb.ptr, sino + (size_t)s0 * P * D, q);
         check_launch("adj_reduce_kernel");
     }
 }
