@@ -438,9 +438,13 @@ def main():
         env = dict(os.environ, HIP_VISIBLE_DEVICES="0")
         base_cfg = risa_amd.config_path("c1")
         with tempfile.TemporaryDirectory() as tmp:
+            # D2H's reconstruction rate over windows of 1000 images (the
+            # reference's 10000 exceeds one run of 4000 frames)
+            plain_cfg = Path(tmp) / "c1_plain.cfg"
+            plain_cfg.write_text(base_cfg.read_text() + '\nrateInterval = 1000\n')
             mixed_cfg = Path(tmp) / "c1_mixed.cfg"
-            mixed_cfg.write_text(base_cfg.read_text() + '\nprecision = "mixed"\n')
-            for label, cfg in (("fp32", base_cfg), ("mixed", mixed_cfg)):
+            mixed_cfg.write_text(base_cfg.read_text() + '\nrateInterval = 1000\nprecision = "mixed"\n')
+            for label, cfg in (("fp32", plain_cfg), ("mixed", mixed_cfg)):
                 try:
                     r = subprocess.run([str(exe), str(cfg), "--frames", str(args.stream_frames), "--unique", "16"],
                                        capture_output=True, text=True, timeout=300, env=env)

@@ -40,6 +40,18 @@ public:
         return item;
     }
 
+    // Takes the front item if there is one, without waiting.
+    bool try_take(T& out)
+    {
+        std::unique_lock<std::mutex> lock(m_);
+        if (q_.empty()) return false;
+        out = std::move(q_.front());
+        q_.pop();
+        lock.unlock();
+        not_full_.notify_one();
+        return true;
+    }
+
     std::size_t size() const
     {
         std::lock_guard<std::mutex> lock(m_);

@@ -339,27 +339,63 @@ BackprojectionMixed::BackprojectionMixed(const std::string& configFile)
     g_ = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_backProjection");
     const int devices = device_count(), lanes = processors_per_device(configFile);
+    // frames per kernel launch: the FP16 backprojector shares each pixel's
+    // weights across the 4 slices of its sample layout, so it takes up to
+    // `backProjectionBatch` queued frames at once (optional key, default 4)
+    batch_ = std::max(1, std::min(config_int_or(configFile, "backProjectionBatch", 4), std::max(1, pool)));
+    const std::size_t plane = (std::size_t)g_.numberOfParallelProjections * g_.numberOfParallelDetectors;
     for (int w = 0; w < devices * lanes; w++) {
         risa_bp_t h = nullptr;
         check(risa_hip_bp_create(&g_, w / lanes, &h), "BackprojectionMixed");
         h_.push_back(h);
         glados::hip::ScopedDevice sd(w / lanes);
         int variant = 0;
-        if (w % lanes == 0) check(risa_hip_bp_autotune_f16(h, 1, nullptr), "BackprojectionMixed autotune");
+        if (w % lanes == 0) check(risa_hip_bp_autotune_f16(h, batch_, nullptr), "BackprojectionMixed autotune");
         check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "BackprojectionMixed variant");
         check(risa_hip_bp_set_variant(h, variant), "BackprojectionMixed variant");
+        // per processor: contiguous staging for a batch of sinograms and images
+        uint16_t* in16 = nullptr;
+        float* out32 = nullptr;
+        if (batch_ > 1) {
+            hip_check(hipMalloc(&in16, (std::size_t)batch_ * plane * sizeof(uint16_t)), "BackprojectionMixed staging");
+            hip_check(hipMalloc(&out32, (std::size_t)batch_ * img_elems(g_) * sizeof(float)),
+                      "BackprojectionMixed staging");
+        }
+        in16_.push_back(in16);
+        out32_.push_back(out32);
     }
-    start("BackprojectionMixed", devices, lanes, pool, img_elems(g_),
-          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
-              check(risa_hip_bp_run_f16(h_[(std::size_t)w], in.data(), kMixedSinoScale, 1, out.data(), s),
-                    "BackprojectionMixed::processor");
-          });
+    start_batched("BackprojectionMixed", devices, lanes, pool, img_elems(g_), batch_,
+                  [this, plane](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      const risa_bp_t h = h_[(std::size_t)w];
+                      if (n == 1) {
+                          check(risa_hip_bp_run_f16(h, ins[0].data(), kMixedSinoScale, 1, outs[0].data(), s),
+                                "BackprojectionMixed::processor");
+                          return;
+                      }
+                      uint16_t* in16 = in16_[(std::size_t)w];
+                      float* out32 = out32_[(std::size_t)w];
+                      const std::size_t img = img_elems(g_);
+                      for (int i = 0; i < n; i++)
+                          hip_check(hipMemcpyAsync(in16 + (std::size_t)i * plane, ins[i].data(),
+                                                   plane * sizeof(uint16_t), hipMemcpyDeviceToDevice, s),
+                                    "BackprojectionMixed gather");
+                      check(risa_hip_bp_run_f16(h, in16, kMixedSinoScale, n, out32, s),
+                            "BackprojectionMixed::processor");
+                      for (int i = 0; i < n; i++)
+                          hip_check(hipMemcpyAsync(outs[i].data(), out32 + (std::size_t)i * img, img * sizeof(float),
+                                                   hipMemcpyDeviceToDevice, s),
+                                    "BackprojectionMixed scatter");
+                  });
 }
 
 BackprojectionMixed::~BackprojectionMixed()
 {
     stop();
     for (auto h : h_) risa_hip_bp_destroy(h);
+    for (auto p : in16_)
+        if (p) (void)hipFree(p);
+    for (auto p : out32_)
+        if (p) (void)hipFree(p);
 }
 
 // ------------------------------------------------------- Backprojection --

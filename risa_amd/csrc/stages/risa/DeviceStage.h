@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <exception>
 #include <functional>
 #include <memory>
@@ -91,6 +92,9 @@ public:
 protected:
     // fn(worker, device, in, out, stream): worker = device * lanes + lane
     using compute_fn = std::function<void(int worker, int dev, input_type& in, output_type& out, hipStream_t s)>;
+    // batched form: n frames at once, ins[i] -> outs[i]
+    using batch_fn = std::function<void(int worker, int dev, input_type* ins, output_type* outs, int n,
+                                        hipStream_t s)>;
 
     DeviceStage() = default;
 
@@ -128,6 +132,18 @@ protected:
         for (auto& w : workers_) w->thread = std::thread(&DeviceStage::loop, this, w.get());
     }
 
+    // Like start() for an allocating stage, but a processor takes a frame
+    // and then, without waiting, up to max_batch - 1 more that are already
+    // queued, and hands them to fn together (kernels that share work across
+    // slices).  Frames still leave in the order they were taken.
+    void start_batched(const char* name, int devices, int lanes, int pool_size, std::size_t out_elements,
+                       int max_batch, batch_fn fn)
+    {
+        max_batch_ = std::max(1, std::min(max_batch, pool_size));
+        bfn_ = std::move(fn);
+        start(name, devices, lanes, pool_size, out_elements, nullptr);
+    }
+
     // Joins the processor threads (the sentinel is sent if process() never
     // saw one) and releases streams and pools.  Call from the derived
     // destructor, before its handles go.
@@ -152,6 +168,7 @@ private:
         int pool = -1;
         glados::Queue<input_type>* queue = nullptr;   // shared by the device's lanes
         std::thread thread;
+        bool got_sentinel = false;                    // this lane's end-of-stream item was taken
     };
 
     void finish()
@@ -169,9 +186,16 @@ private:
     {
         try {
             glados::hip::ScopedDevice sd(w->device);
+            if (bfn_) {
+                batched_loop(w);
+                return;
+            }
             for (;;) {
                 input_type in = w->queue->take();
-                if (!in.valid()) break;
+                if (!in.valid()) {
+                    w->got_sentinel = true;
+                    break;
+                }
                 if constexpr (std::is_same<input_type, output_type>::value) {
                     if (!allocating_) {
                         fn_(w->index, w->device, in, in, w->stream);
@@ -193,9 +217,45 @@ private:
             std::lock_guard<std::mutex> lock(err_m_);
             if (!error_) error_ = std::current_exception();
             // drain so producers never block on a dead consumer
-            for (;;) {
+            while (!w->got_sentinel) {
                 input_type in = w->queue->take();
-                if (!in.valid()) break;
+                if (!in.valid()) w->got_sentinel = true;
+            }
+        }
+    }
+
+    void batched_loop(Worker* w)
+    {
+        auto* pool = glados::MemoryPool<typename output_type::manager_type>::instance();
+        std::vector<input_type> ins;
+        std::vector<output_type> outs;
+        bool end = false;
+        while (!end) {
+            ins.clear();
+            outs.clear();
+            input_type first = w->queue->take();
+            if (!first.valid()) {
+                w->got_sentinel = true;
+                break;
+            }
+            ins.push_back(std::move(first));
+            while ((int)ins.size() < max_batch_) {
+                input_type more;
+                if (!w->queue->try_take(more)) break;
+                if (!more.valid()) {   // this lane's sentinel: finish the batch, then stop
+                    w->got_sentinel = end = true;
+                    break;
+                }
+                ins.push_back(std::move(more));
+            }
+            for (std::size_t i = 0; i < ins.size(); i++) outs.push_back(pool->requestMemory(w->pool));
+            bfn_(w->index, w->device, ins.data(), outs.data(), (int)ins.size(), w->stream);
+            hip_check(hipStreamSynchronize(w->stream), name_);
+            for (std::size_t i = 0; i < ins.size(); i++) {
+                outs[i].setIdx(ins[i].index());
+                outs[i].setPlane(ins[i].plane());
+                outs[i].setStart(ins[i].start());
+                results_.push(std::move(outs[i]));
             }
         }
     }
@@ -212,6 +272,8 @@ protected:
 private:
     const char* name_ = "stage";
     compute_fn fn_;
+    batch_fn bfn_;
+    int max_batch_ = 1;
     int lanes_ = 1;
     bool allocating_ = false;
     bool stopped_ = false;

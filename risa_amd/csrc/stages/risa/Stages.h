@@ -137,7 +137,10 @@ public:
 
 private:
     risa_geom g_{};
+    int batch_ = 4;                  // frames per launch (key backProjectionBatch)
     std::vector<risa_bp_t> h_;
+    std::vector<uint16_t*> in16_;    // per processor: [batch][P][D] staging
+    std::vector<float*> out32_;      // per processor: [batch][N][N] staging
 };
 
 // precision = "mixed" in the configuration (default "fp32").
