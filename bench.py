@@ -86,6 +86,8 @@ def kernel_only(args, g, dev):
             if args.kernel == "bp":
                 S = BP_SLICES_IN_FLIGHT
                 bp.autotune_slices(S)
+                if args.variant is not None:
+                    bp.variant = (bp.variant & 0xff00) | args.variant
                 sino = torch.from_numpy((rng.standard_normal((S, P, D)) * 0.01).astype(np.float32)).to(dev)
                 img = torch.empty((S, N, N), dtype=torch.float32, device=dev)
                 fn, per = (lambda: bp.batch(sino, S, img)), S
@@ -93,6 +95,8 @@ def kernel_only(args, g, dev):
             else:
                 B = 32
                 bp.autotune_f16(B)
+                if args.variant is not None:
+                    bp.variant = (bp.variant & 0xff) | (args.variant << 8)
                 sino = torch.from_numpy((rng.standard_normal((B // 4, P, D, 4)) * 2.0).astype(np.float16)).to(dev)
                 img = torch.empty((B, N, N), dtype=torch.float32, device=dev)
                 fn, per = (lambda: bp.f16x4(sino, 256.0, B, img)), B
@@ -185,6 +189,7 @@ def main():
     ap.add_argument("--kernel", choices=("bp", "bp16", "adj", "att", "att_mm", "reorder", "prep", "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
+    ap.add_argument("--variant", type=int, default=None, help="--kernel bp/bp16: this tile variant, not the autotuned one")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
