@@ -93,7 +93,7 @@ def kernel_only(args, g, dev):
                 fn, per = (lambda: bp.batch(sino, S, img)), S
                 what = f"bp FP32, {S} slices {N}^2 x {P} angles per launch, variant {bp.variant & 0xff}"
             else:
-                B = 32
+                B = args.slices or 32
                 bp.autotune_f16(B)
                 if args.variant is not None:
                     bp.variant = (bp.variant & 0xff) | (args.variant << 8)
@@ -108,10 +108,13 @@ def kernel_only(args, g, dev):
                            filterType=0, fusedMasking=1)
             S4, P4, D4, N4 = 16, 720, 512, 512
             bpa = device.Backprojection(ga)
+            if args.adjoint_gather:
+                bpa.adjoint_method(1)
             im4 = torch.from_numpy(rng.standard_normal((S4, N4, N4)).astype(np.float32)).to(dev)
             gs4 = torch.empty((S4, P4, D4), dtype=torch.float32, device=dev)
             fn, per = (lambda: bpa.adjoint_batch(im4, S4, gs4)), S4
-            what = f"adjoint projector, {S4} slices of {N4}^2 -> {P4} x {D4} per call"
+            what = (f"adjoint projector ({'gather' if args.adjoint_gather else 'sheared paths'}), {S4} slices of "
+                    f"{N4}^2 -> {P4} x {D4} per call")
         elif args.kernel in ("att", "att_mm", "reorder"):
             # 32 C1 frames per launch: attenuation of detector-major frames,
             # the same reading ROFEX module-major frames through the fused
@@ -190,6 +193,8 @@ def main():
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     ap.add_argument("--variant", type=int, default=None, help="--kernel bp/bp16: this tile variant, not the autotuned one")
+    ap.add_argument("--slices", type=int, default=None, help="--kernel bp16: slices per launch (default 32)")
+    ap.add_argument("--adjoint-gather", action="store_true", help="--kernel adj: the gather adjoint kernel")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -422,7 +427,7 @@ def main():
                        "forward_us_per_slice": round(times["forward"] * 1e3 / S4, 1),
                        "backward_us_per_slice": round(times["backward"] * 1e3 / S4, 1),
                        "what": "C4 geometry; forward = filter + batched backprojection; backward = "
-                               "bp_adjoint_path_kernel (batched, 2 slices per pass) + fft_grad_taps_kernel + "
+                               "bp_adjoint_path_kernel (batched, 8 slices per pass) + fft_grad_taps_kernel + "
                                "conjugate filter; one step of the autograd function"}
             log(f"differentiable FBP: {adjoint}")
         except Exception as exc:  # reported, not fatal

@@ -302,14 +302,20 @@ int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_scale, int s
  * risa_hip_filter_run_f16x4 (4 slices interleaved per detector sample). */
 int risa_hip_bp_run_f16x4(risa_bp_t h, const uint16_t* sino_x4, float in_scale, int slices,
                           float* img, void* stream);
-/* Adjoint of risa_hip_bp_run (linear): the projector that is its transpose
- * (mask included when fused).  img [N][N] -> sino [P][D].  For D <= N
- * (scale <= 1) a sheared-path kernel, else a ray-driven gather.  (No
- * reference counterpart: the differentiable path's backward.) */
+/* Adjoint of risa_hip_bp_run (linear): the exact transpose of the FP32
+ * linear backprojector with the handle's current tile variant (its float
+ * coordinates and tile origins are recomputed; mask included when fused).
+ * img [N][N] -> sino [P][D].  For D <= N (scale <= 1) a sheared-path kernel,
+ * else a ray-driven gather.  (No reference counterpart: the differentiable
+ * path's backward.) */
 int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream);
 /* The same for `slices` images [slices][N][N] -> [slices][P][D] in passes of
- * up to 2 slices per launch sequence through buffers sized at create. */
+ * up to 8 slices per launch sequence through buffers sized at create. */
 int risa_hip_bp_adjoint_run_batch(risa_bp_t h, const float* img, int slices, float* sino, void* stream);
+/* Adjoint method: 0 (default) the sheared-path kernel where it applies,
+ * 1 the ray-driven gather everywhere (same operator, slower; for comparison
+ * and profiling).  No reference counterpart. */
+int risa_hip_bp_adjoint_method(risa_bp_t h, int method);
 /* Kernel (tile-shape) variants: bits 0-7 the FP32 variant, bits 8-15 the
  * FP16 one; both start at 0 and are chosen by the autotune calls below. */
 int risa_hip_bp_set_variant(risa_bp_t h, int variant);

@@ -182,6 +182,7 @@ _SIGS = {
     "risa_hip_bp_run_f16x4": [P, P, C.c_float, C.c_int, P, P],
     "risa_hip_bp_adjoint_run": [P, P, P, P],
     "risa_hip_bp_adjoint_run_batch": [P, P, C.c_int, P, P],
+    "risa_hip_bp_adjoint_method": [P, C.c_int],
     "risa_hip_bp_set_variant": [P, C.c_int],
     "risa_hip_bp_get_variant": [P, C.POINTER(C.c_int)],
     "risa_hip_bp_autotune": [P, P],

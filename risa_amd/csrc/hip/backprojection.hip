@@ -1321,6 +1321,7 @@ struct risa_bp_s {
     risa::DeviceBuffer<float> imgT;  // transposed image for the fallback adjoint [N][N]
     // sheared-path adjoint (scale <= 1): tables and per-pass buffers
     bool adj_path = false;
+    bool adj_gather = false;                 // risa_hip_bp_adjoint_method(h, 1)
     risa::AdjParams adj{};
     int adj_groups = 0;                      // workgroups along the sorted angle list
     risa::DeviceBuffer<risa::AdjAngle> adj_angles;
@@ -1513,7 +1514,7 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
             const size_t plane = (size_t)g->numberOfParallelProjections * g->numberOfParallelDetectors;
             h->x4.resize((size_t)kF16StageGroups * plane);
             setup_adjoint(h);
-            if (!h->adj_path) h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);
+            h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);   // the gather adjoint's transpose
         } catch (...) {
             delete h;
             throw;
@@ -1656,7 +1657,7 @@ static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, 
 {
     const int N = h->g.numberOfPixels;
     const long P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
-    if (!h->adj_path) {
+    if (!h->adj_path || h->adj_gather) {
         for (int i = 0; i < slices; i++) adjoint_gather(h, img + (size_t)i * N * N, sino + (size_t)i * P * D, s);
         return;
     }
@@ -1701,6 +1702,15 @@ extern "C" int risa_hip_bp_adjoint_run_batch(risa_bp_t h, const float* img, int 
     return guarded("risa_hip_bp_adjoint_run_batch", [&] {
         if (!h || !img || !sino || slices < 0) fail(RISA_ERR_ARG, "bad argument");
         if (slices) adjoint_run(h, img, slices, sino, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_bp_adjoint_method(risa_bp_t h, int method)
+{
+    return guarded("risa_hip_bp_adjoint_method", [&] {
+        if (!h) fail(RISA_ERR_ARG, "null handle");
+        if (method != 0 && method != 1) fail(RISA_ERR_ARG, "adjoint method must be 0 or 1");
+        h->adj_gather = method == 1;
     });
 }
 

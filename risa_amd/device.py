@@ -250,6 +250,10 @@ class Backprojection(_Handle):
         check(load().risa_hip_bp_adjoint_run_batch(self.h, ptr(img), slices, ptr(sino), _stream()))
         return sino
 
+    def adjoint_method(self, method: int):
+        """0: sheared-path kernel where it applies (default); 1: gather kernel."""
+        check(load().risa_hip_bp_adjoint_method(self.h, method))
+
     def autotune(self):
         check(load().risa_hip_bp_autotune(self.h, _stream()))
 

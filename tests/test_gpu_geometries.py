@@ -342,6 +342,33 @@ def test_adjoint_weights_match_rounding_model(risa, geom_c0, cuda, N, P, D, rot,
     bp.close()
 
 
+@pytest.mark.parametrize("N,P,D,rot,mask", [(257, 181, 200, 13.0, 1), (512, 120, 512, 0.0, 1)])
+def test_adjoint_methods_agree(risa, geom_c0, cuda, N, P, D, rot, mask):
+    """The sheared-path adjoint and the gather adjoint (method 1) are the same
+    operator -- both recompute the forward's coordinates -- so they agree to
+    float summation order (rel RMSE <= 1e-6)."""
+    import torch
+    from risa_amd import device
+
+    print(f"case: adjoint methods N={N} P={P} D={D} rot={rot} mask={mask}", flush=True)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=P, numberOfParallelDetectors=D,
+                        rotationOffset=rot, fusedMasking=mask, filterType=0)
+    rng = np.random.default_rng(N * P)
+    S = 3
+    x = _t(rng.standard_normal((S, N, N)).astype(np.float32), cuda)
+    bp = device.Backprojection(g)
+    path = torch.empty((S, P, D), dtype=torch.float32, device=cuda)
+    gath = torch.empty_like(path)
+    bp.adjoint_batch(x, S, path)
+    bp.adjoint_method(1)
+    bp.adjoint_batch(x, S, gath)
+    _sync()
+    err = rel_rmse(gath.cpu().numpy(), path.cpu().numpy())
+    print(f"  rel RMSE gather vs sheared paths {err:.2e}", flush=True)
+    assert err <= 1e-6
+    bp.close()
+
+
 @pytest.mark.parametrize("N,P,D,rot,mask", [(257, 181, 200, 13.0, 1), (128, 90, 128, -30.0, 0)])
 def test_adjoint_dot_every_tile_variant(risa, orc, geom_c0, cuda, N, P, D, rot, mask):
     """The adjoint is the transpose of the forward tile variant in use (its
