@@ -137,6 +137,31 @@ def kernel_only(args, g, dev):
                 fn = lambda: att.batch(raw, planes, out)
                 layout = "module-major (reordering fused)" if args.kernel == "att_mm" else "detector-major"
                 what = f"attenuation, {F} frames of {Pf} x {Df}, {layout} input"
+        elif args.kernel in ("rebin", "dft"):
+            # the mixed chain's front stages at its 64-frame shape (C1, both
+            # planes): rebinning of attenuated FP32 frames, and the DFT-GEMM
+            # filter to the FP16 x4 layout
+            from risa_amd import synth
+            F, Pf, Df = 64, g.numberOfFanProjections, g.numberOfFanDetectors
+            P, D = g.numberOfParallelProjections, g.numberOfParallelDetectors
+            planes = np.arange(F, dtype=np.int32) % g.numberOfPlanes
+            per = F
+            if args.kernel == "dft":
+                filt = device.Filter(g, mode=risa_amd.FILTER_MODE_GEMM_SPLIT)
+                par = torch.from_numpy((rng.standard_normal((F, P, D)) * 0.05).astype(np.float32)).to(dev)
+                x4 = torch.empty((F // 4, P, D, 4), dtype=torch.float16, device=dev)
+                fn = lambda: filt.f16x4(par, x4, F, 256.0)
+                what = f"DFT-GEMM filter (hi+lo FP16 MFMA, FP32 in, FP16 x4 out), {F} slices of {P} x {D}"
+            else:
+                avg_ref, avg_dark = synth.prepare_reference(g, synth.reference_frames(g, 4, seed=901))
+                raw = torch.from_numpy(rng.integers(100, 60000, size=(F, Pf, Df), dtype=np.uint16)).to(dev)
+                att = device.Attenuation(g, avg_ref, avg_dark)
+                f2p = device.Fan2Para(g)
+                par = torch.empty((F, P, D), dtype=torch.float32, device=dev)
+                fan = torch.empty((F, Pf, Df), dtype=torch.float32, device=dev)
+                att.batch(raw, planes, fan)
+                fn = lambda: f2p.batch(fan, planes, par)
+                what = f"rebinning, {F} attenuated FP32 frames of {Pf} x {Df} -> {P} x {D}"
         elif args.kernel == "prep":
             # flat-field preparation of 8 reference frames per plane (1 % dead
             # columns): defect detection and repair of each, then the average
@@ -189,7 +214,8 @@ def main():
     ap.add_argument("--stream-frames", type=int, default=4000,
                     help="frame-planes through the stage pipeline (C1: 2 planes x 2000 frames)")
     ap.add_argument("--no-c0", action="store_true", help="skip the C0 GPU / CPU leg")
-    ap.add_argument("--kernel", choices=("bp", "bp16", "adj", "att", "att_mm", "reorder", "prep", "copy"),
+    ap.add_argument("--kernel", choices=("bp", "bp16", "adj", "att", "att_mm", "reorder", "rebin", "dft", "prep",
+                                         "copy"),
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     ap.add_argument("--variant", type=int, default=None, help="--kernel bp/bp16: this tile variant, not the autotuned one")
