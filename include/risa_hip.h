@@ -322,7 +322,8 @@ int risa_hip_bp_set_variant(risa_bp_t h, int variant);
 int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
 /* Tile shape of a variant (f16 = 0: FP32 kernel, 1: FP16 kernel):
  * shape = {tile x, tile y, pixels per thread, staged entries per thread,
- * angles per step, groups of 4 slices (1 for FP32), max angles per chunk};
+ * angles per step, slices per workgroup (FP16: groups of 4 slices; FP32: 1 or
+ * 2, two slices of a batch sharing each pixel's coordinate), max angles per chunk};
  * RISA_ERR_ARG past the last variant.  (Tooling; no reference counterpart.) */
 int risa_hip_bp_variant_shape(int f16, int variant, int shape[7]);
 /* Times every variant that fits the geometry and keeps the fastest: FP32
@@ -330,8 +331,16 @@ int risa_hip_bp_variant_shape(int f16, int variant, int shape[7]);
 int risa_hip_bp_autotune(risa_bp_t h, void* stream);
 /* The same, timing the FP32 variants with `slices` slices per launch (the
  * number of frames a caller keeps in flight) and the FP16 variants with
- * max(8, slices).  (Autotuning has no reference counterpart.) */
+ * max(8, slices).  One-slice-per-workgroup FP32 variants only: for callers
+ * that launch one slice at a time.  (Autotuning has no reference
+ * counterpart.) */
 int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream);
+/* FP32 variants only, timed at `slices` slices per risa_hip_bp_run_batch
+ * launch, including those that reconstruct two slices per workgroup. */
+int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream);
+/* Slices per workgroup of the handle's FP32 variant (1 or 2): a caller
+ * batching frames gains from launching that many at a time. */
+int risa_hip_bp_slices_per_tile(risa_bp_t h, int* slices);
 /* FP16 variants only, timed at `slices` slices per risa_hip_bp_run_f16x4
  * launch (the batch the caller runs). */
 int risa_hip_bp_autotune_f16(risa_bp_t h, int slices, void* stream);
@@ -382,8 +391,8 @@ int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float*
 int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
                        float* img, int use_graph, void* stream);
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
- * 2 filtered (PLAIN: FP32, one frame slot per lane; MIXED: FP16 4-slice
- * layout, one slot per frame). */
+ * 2 filtered (PLAIN: FP32, two frame slots per lane, frame 0 of a run in
+ * slot 0; MIXED: FP16 4-slice layout, one slot per frame). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces
  * the autotuner's pick).  A change synchronises the chain's device and

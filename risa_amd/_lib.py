@@ -187,6 +187,8 @@ _SIGS = {
     "risa_hip_bp_get_variant": [P, C.POINTER(C.c_int)],
     "risa_hip_bp_autotune": [P, P],
     "risa_hip_bp_autotune_slices": [P, C.c_int, P],
+    "risa_hip_bp_autotune_batch": [P, C.c_int, P],
+    "risa_hip_bp_slices_per_tile": [P, C.POINTER(C.c_int)],
     "risa_hip_bp_autotune_f16": [P, C.c_int, P],
     "risa_hip_bp_variant_shape": [C.c_int, C.c_int, C.POINTER(C.c_int)],
     "risa_hip_bp_destroy": [P],

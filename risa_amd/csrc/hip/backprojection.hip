@@ -166,24 +166,34 @@ __device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, flo
 // (zeros outside [0, D)), so that s[k] + (t' - k') d = e0 + t' d and the
 // update needs no fraction: acc += e0; acc = fma(t', d, acc).
 //
-// One TILE x TILE tile of one slice, shared by the per-slice kernel and the
+// One tile of SB slices (1 or 2), shared by the per-slice kernels and the
 // persistent uneven-batch kernel.  All threads of the workgroup call it
-// together.  in_f16: the sinogram is FP16 (value * in_scale).
-template <int TX, int TY, int PPT, int E, int U>
+// together.  in_f16: the sinogram is FP16 (value * in_scale; SB = 1 only).
+// With SB = 2 the workgroup reconstructs the same tile of two slices of one
+// geometry (sino_v / I and sino2 / I2; sino2 null: only the first): a staged
+// entry holds both slices' pairs (e0, d, e0', d'), so one 16-byte LDS read
+// and one coordinate, floor and address serve two updates, and each slice's
+// arithmetic is exactly the one-slice tile's.
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+template <int TX, int TY, int PPT, int E, int U, int SB = 1>
 __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* __restrict__ I,
                                         const float* __restrict__ cosT, const float* __restrict__ sinT,
-                                        const BpParams& p, int x0, int y0, int in_f16, char* smem)
+                                        const BpParams& p, int x0, int y0, int in_f16, char* smem,
+                                        const float* __restrict__ sino2 = nullptr, float* __restrict__ I2 = nullptr)
 {
+    static_assert(SB == 1 || SB == 2, "one or two slices per tile");
     constexpr int NT = TX * TY / PPT;
     constexpr int RS = TY / PPT;
+    constexpr unsigned EB = 8u * SB;   // bytes per staged entry
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const float* S = reinterpret_cast<const float*>(sino_v);
     const uint16_t* S16 = reinterpret_cast<const uint16_t*>(sino_v);
+    const bool two = SB == 2 && sino2 != nullptr;
     const int W = p.W, AE = p.AE;
-    float2* rows = reinterpret_cast<float2*>(smem);
+    char* rows = smem;
     const unsigned rbase = lds_address(rows);
-    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * sizeof(float2));
+    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * EB);
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int x = x0 + tx;
@@ -203,14 +213,19 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
         for (int r = 0; r < PPT; r++) {
             const int y = y0 + ty + r * RS;
-            if (x < p.N && y < p.N) I[(long)y * p.N + x] = p.maskingValue;
+            if (x < p.N && y < p.N) {
+                I[(long)y * p.N + x] = p.maskingValue;
+                if (two) I2[(long)y * p.N + x] = p.maskingValue;
+            }
         }
         return;
     }
     const bool wave_active = __any(any);
-    float acc[PPT];
+    float acc[PPT][SB];
 #pragma unroll
-    for (int r = 0; r < PPT; r++) acc[r] = 0.0f;
+    for (int r = 0; r < PPT; r++)
+#pragma unroll
+        for (int b = 0; b < SB; b++) acc[r][b] = 0.0f;
 
     const float xa = ((float)x0 - p.center) * p.scale, xb = ((float)(x0 + TX - 1) - p.center) * p.scale;
     const float ya = ((float)y0 - p.center) * p.scale, yb = ((float)(y0 + TY - 1) - p.center) * p.scale;
@@ -225,7 +240,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
             const int row = (ch & 1) * AE * W + a * W;
-            inf[a] = make_float4(w.c, w.s, w.negT, __uint_as_float(rbase + (unsigned)((row - w.kmin) * 8)));
+            inf[a] = make_float4(w.c, w.s, w.negT, __uint_as_float(rbase + (unsigned)(row - w.kmin) * EB));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
     };
@@ -233,7 +248,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     make_info(1);
     __syncthreads();
 
-    float s0r[E], s1r[E];
+    float s0r[E][SB], s1r[E][SB];
     bool in0r[E], in1r[E];
     int kr[E];
     auto fetch = [&](int ch) {
@@ -254,19 +269,26 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 const unsigned row = __umul24((unsigned)(p0 + a), (unsigned)p.D);   // P, D < 2^24
                 const unsigned i0 = row + (unsigned)clamp_index(k, p.D - 1);
                 const unsigned i1 = row + (unsigned)clamp_index(k + 1, p.D - 1);
-                float v0, v1;
-                if (in_f16) {
+                if (SB == 1 && in_f16) {
                     const char* b = reinterpret_cast<const char*>(S16);
-                    v0 = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i0 * 2u));
-                    v1 = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i1 * 2u));
+                    s0r[q][0] = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i0 * 2u));
+                    s1r[q][0] = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i1 * 2u));
                 } else {
                     const char* b = reinterpret_cast<const char*>(S);
-                    v0 = *reinterpret_cast<const float*>(b + i0 * 4u);
-                    v1 = *reinterpret_cast<const float*>(b + i1 * 4u);
+                    s0r[q][0] = *reinterpret_cast<const float*>(b + i0 * 4u);
+                    s1r[q][0] = *reinterpret_cast<const float*>(b + i1 * 4u);
+                    if (SB == 2) {
+                        if (two) {
+                            const char* b2 = reinterpret_cast<const char*>(sino2);
+                            s0r[q][SB - 1] = *reinterpret_cast<const float*>(b2 + i0 * 4u);
+                            s1r[q][SB - 1] = *reinterpret_cast<const float*>(b2 + i1 * 4u);
+                        } else {
+                            s0r[q][SB - 1] = 0.0f;
+                            s1r[q][SB - 1] = 0.0f;
+                        }
+                    }
                 }
                 // zeroed at commit, so the loads stay in flight meanwhile
-                s0r[q] = v0;
-                s1r[q] = v1;
                 in0r[q] = (unsigned)k < (unsigned)p.D;
                 in1r[q] = (unsigned)(k + 1) < (unsigned)p.D;
             }
@@ -274,16 +296,31 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     };
     auto commit = [&](int ch) {
         const int nae = min(AE, p.P - ch * AE) * W;
-        float2* buf = rows + (ch & 1) * AE * W;
+        char* buf = rows + (size_t)(ch & 1) * AE * W * EB;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
             if (e < nae) {
-                const float s0 = in0r[q] ? s0r[q] : 0.0f, s1 = in1r[q] ? s1r[q] : 0.0f;
-                const float d = s1 - s0;
-                buf[e] = make_float2(s0 - (float)kr[q] * d, d);
+                float ent[2 * SB];
+#pragma unroll
+                for (int b = 0; b < SB; b++) {
+                    const float s0 = in0r[q] ? s0r[q][b] : 0.0f, s1 = in1r[q] ? s1r[q][b] : 0.0f;
+                    const float d = s1 - s0;
+                    ent[2 * b] = s0 - (float)kr[q] * d;
+                    ent[2 * b + 1] = d;
+                }
+                if (SB == 1)
+                    reinterpret_cast<float2*>(buf)[e] = make_float2(ent[0], ent[1]);
+                else
+                    reinterpret_cast<float4*>(buf)[e] = make_float4(ent[0], ent[1], ent[2 * SB - 2], ent[2 * SB - 1]);
             }
         }
+    };
+    using LdsEnt = typename std::conditional<SB == 1, lds_f2, lds_f4>::type;
+    // one update of slice b from the entry q at coordinate t
+    auto update = [&](float& a, const LdsEnt& q, int b, float t) {
+        a += q[2 * b];
+        a = __builtin_fmaf(t, q[2 * b + 1], a);
     };
 
     fetch(0);
@@ -300,39 +337,36 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             for (; a + U <= na; a += U) {
                 float4 in[U];
                 float t[U][PPT];
-                float2 v[U][PPT];
+                LdsEnt v[U][PPT];
 #pragma unroll
                 for (int u = 0; u < U; u++) in[u] = inf[a + u];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const float xc = __builtin_fmaf(xp, in[u].x, in[u].z);
-                    const unsigned b = __float_as_uint(in[u].w);
+                    const unsigned bb = __float_as_uint(in[u].w);
 #pragma unroll
                     for (int r = 0; r < PPT; r++) {
                         t[u][r] = __builtin_fmaf(yp[r], in[u].y, xc);
-                        const lds_f2 q = lds_load<lds_f2>(b + (unsigned)floor_to_int(t[u][r]) * 8u);
-                        v[u][r] = make_float2(q.x, q.y);
+                        v[u][r] = lds_load<LdsEnt>(bb + (unsigned)floor_to_int(t[u][r]) * EB);
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++)
 #pragma unroll
-                    for (int r = 0; r < PPT; r++) {
-                        acc[r] += v[u][r].x;
-                        acc[r] = __builtin_fmaf(t[u][r], v[u][r].y, acc[r]);
-                    }
+                    for (int r = 0; r < PPT; r++)
+#pragma unroll
+                        for (int b = 0; b < SB; b++) update(acc[r][b], v[u][r], b, t[u][r]);
             }
             for (; a < na; a++) {
                 const float4 in = inf[a];
                 const float xc = __builtin_fmaf(xp, in.x, in.z);
-                const unsigned b = __float_as_uint(in.w);
+                const unsigned bb = __float_as_uint(in.w);
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
                     const float t = __builtin_fmaf(yp[r], in.y, xc);
-                    const lds_f2 q = lds_load<lds_f2>(b + (unsigned)floor_to_int(t) * 8u);
-                    const float2 v = make_float2(q.x, q.y);
-                    acc[r] += v.x;
-                    acc[r] = __builtin_fmaf(t, v.y, acc[r]);
+                    const LdsEnt q = lds_load<LdsEnt>(bb + (unsigned)floor_to_int(t) * EB);
+#pragma unroll
+                    for (int b = 0; b < SB; b++) update(acc[r][b], q, b, t);
                 }
             }
         }
@@ -343,22 +377,35 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
     for (int r = 0; r < PPT; r++) {
         const int y = y0 + ty + r * RS;
-        if (x < p.N && y < p.N) I[(long)y * p.N + x] = use[r] ? acc[r] * p.norm * p.out_scale : p.maskingValue;
+        if (x < p.N && y < p.N) {
+            I[(long)y * p.N + x] = use[r] ? acc[r][0] * p.norm * p.out_scale : p.maskingValue;
+            if (two) I2[(long)y * p.N + x] = use[r] ? acc[r][SB - 1] * p.norm * p.out_scale : p.maskingValue;
+        }
     }
 }
 
-template <int TX, int TY, int PPT, int E, int U>
+// Workgroup (x, y): tile x of slices SB*y .. SB*y + SB - 1 (those below `slices`).
+template <int TX, int TY, int PPT, int E, int U, int SB>
 __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __restrict__ sino,
                                                                  float* __restrict__ img,
                                                                  const float* __restrict__ cosT,
-                                                                 const float* __restrict__ sinT, BpParams p)
+                                                                 const float* __restrict__ sinT, int slices,
+                                                                 BpParams p)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int2 txy = bp_tile_xy(blockIdx.x, p);
     const int x0 = txy.x * TX, y0 = txy.y * TY;
-    const long slice = blockIdx.y;
-    bp_tile<TX, TY, PPT, E, U>(sino + slice * (long)p.P * p.D, img + slice * (long)p.N * p.N, cosT, sinT, p, x0,
-                               y0, 0, smem);
+    const long slice = (long)blockIdx.y * SB;
+    const long sino_stride = (long)p.P * p.D, img_stride = (long)p.N * p.N;
+    const float* s0 = sino + slice * sino_stride;
+    float* i0 = img + slice * img_stride;
+    if (SB == 1) {
+        bp_tile<TX, TY, PPT, E, U, 1>(s0, i0, cosT, sinT, p, x0, y0, 0, smem);
+    } else {
+        const bool two = slice + 1 < slices;
+        bp_tile<TX, TY, PPT, E, U, SB>(s0, i0, cosT, sinT, p, x0, y0, 0, smem, two ? s0 + sino_stride : nullptr,
+                                       two ? i0 + img_stride : nullptr);
+    }
 }
 
 // A job of the persistent uneven batch (device copy of risa_bp_job + derived
@@ -1248,16 +1295,20 @@ namespace risa {
 // and the cap on angles per chunk.  The chunk holds AE = min(amax,
 // E * threads / W) angles so one thread stages at most E entries.
 struct BpVariant {
-    int tx, ty, ppt, e, u, amax;
-    void (*fn)(const float*, float*, const float*, const float*, BpParams);
+    int tx, ty, ppt, e, u, sb, amax;   // sb: slices per workgroup (1 or 2)
+    void (*fn)(const float*, float*, const float*, const float*, int, BpParams);
 };
 
-#define RISA_BPV(TX, TY, PPT, E, U, AMAX) {TX, TY, PPT, E, U, AMAX, bp_linear_kernel<TX, TY, PPT, E, U>}
+#define RISA_BPV(TX, TY, PPT, E, U, SB, AMAX) {TX, TY, PPT, E, U, SB, AMAX, bp_linear_kernel<TX, TY, PPT, E, U, SB>}
 static const BpVariant kVariants[] = {
-    RISA_BPV(32, 32, 4, 8, 4, 64), RISA_BPV(32, 16, 2, 8, 4, 64), RISA_BPV(16, 16, 1, 4, 4, 64),
-    RISA_BPV(32, 32, 8, 16, 2, 64), RISA_BPV(64, 16, 4, 16, 4, 64), RISA_BPV(32, 16, 4, 16, 4, 64),
-    RISA_BPV(16, 32, 2, 8, 4, 64), RISA_BPV(32, 32, 4, 8, 2, 64), RISA_BPV(16, 16, 2, 8, 4, 64),
-    RISA_BPV(32, 8, 1, 4, 4, 64),
+    RISA_BPV(32, 32, 4, 8, 4, 1, 64), RISA_BPV(32, 16, 2, 8, 4, 1, 64), RISA_BPV(16, 16, 1, 4, 4, 1, 64),
+    RISA_BPV(32, 32, 8, 16, 2, 1, 64), RISA_BPV(64, 16, 4, 16, 4, 1, 64), RISA_BPV(32, 16, 4, 16, 4, 1, 64),
+    RISA_BPV(16, 32, 2, 8, 4, 1, 64), RISA_BPV(32, 32, 4, 8, 2, 1, 64), RISA_BPV(16, 16, 2, 8, 4, 1, 64),
+    RISA_BPV(32, 8, 1, 4, 4, 1, 64),
+    // two slices per workgroup: one coordinate, floor, address and 16-byte LDS read per two updates;
+    // half the staged entries per thread keep the doubled entries at ~33 KB of LDS per workgroup
+    RISA_BPV(32, 32, 4, 4, 2, 2, 64), RISA_BPV(32, 32, 4, 4, 4, 2, 64), RISA_BPV(32, 16, 2, 4, 4, 2, 64),
+    RISA_BPV(16, 16, 1, 2, 4, 2, 64), RISA_BPV(16, 16, 2, 4, 2, 2, 64), RISA_BPV(32, 32, 4, 6, 2, 2, 64),
 };
 #undef RISA_BPV
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -1528,11 +1579,11 @@ static void launch_linear(risa_bp_t h, int variant, const float* sino, float* im
     const BpVariant& v = kVariants[variant];
     BpParams p = make_params(h, v.tx, v.ty);
     const int threads = v.tx * v.ty / v.ppt;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8);
-    const size_t lds = tile_lds(p.AE, p.W, sizeof(float2));
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8 * v.sb);
+    const size_t lds = tile_lds(p.AE, p.W, 8 * v.sb);
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
-    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), slices);
-    hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, p);
+    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.sb));
+    hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_kernel");
 }
 
@@ -1738,7 +1789,7 @@ extern "C" int risa_hip_bp_variant_shape(int f16, int variant, int shape[7])
         } else {
             if (variant >= kNumVariants) fail(RISA_ERR_ARG, "variant out of range");
             const BpVariant& v = kVariants[variant];
-            const int s[7] = {v.tx, v.ty, v.ppt, v.e, v.u, 1, v.amax};
+            const int s[7] = {v.tx, v.ty, v.ppt, v.e, v.u, v.sb, v.amax};
             std::copy(s, s + 7, shape);
         }
     });
@@ -1755,7 +1806,9 @@ extern "C" int risa_hip_bp_get_variant(risa_bp_t h, int* variant)
 // Times every variant that fits the geometry with HIP events and keeps the
 // fastest: FP32 variants at s32 slices per launch (the frames a caller keeps
 // in flight), FP16 variants at s16 slices per launch; 0 skips a precision.
-static void autotune(risa_bp_t h, int s32, int s16, hipStream_t s)
+// max_sb: the FP32 variants considered take at most this many slices per
+// workgroup (1 for callers that launch one slice at a time).
+static void autotune(risa_bp_t h, int s32, int s16, hipStream_t s, int max_sb = 1)
 {
     DeviceGuard dg(h->device);
     const int N = h->g.numberOfPixels, P = h->g.numberOfParallelProjections, D = h->g.numberOfParallelDetectors;
@@ -1783,6 +1836,7 @@ static void autotune(risa_bp_t h, int s32, int s16, hipStream_t s)
         std::vector<float> t32(kNumVariants, 1e30f), t16(kNumF16Variants, 1e30f);
         for (int round = 0; round < 2; round++) {
             for (int v = 0; s32 > 0 && v < kNumVariants; v++) {
+                if (kVariants[v].sb > max_sb) continue;
                 try {
                     t32[v] = std::min(t32[v], time_it([&] { launch_linear(h, v, sino.ptr, img.ptr, s32, s); }));
                 } catch (const Error& e) {
@@ -1816,6 +1870,22 @@ extern "C" int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream
     return guarded("risa_hip_bp_autotune_slices", [&] {
         if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
         autotune(h, slices, std::max(8, slices), as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream)
+{
+    return guarded("risa_hip_bp_autotune_batch", [&] {
+        if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
+        autotune(h, slices, 0, as_stream(stream), 2);
+    });
+}
+
+extern "C" int risa_hip_bp_slices_per_tile(risa_bp_t h, int* slices)
+{
+    return guarded("risa_hip_bp_slices_per_tile", [&] {
+        if (!h || !slices) fail(RISA_ERR_ARG, "null argument");
+        *slices = kVariants[h->variant].sb;
     });
 }
 

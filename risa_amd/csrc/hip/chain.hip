@@ -6,10 +6,12 @@
 // stream-ordered, and a captured hipGraph replays the launch sequence.
 //
 // PLAIN: frame by frame, FP32 throughout (FFT filter, FP32 backprojector).
-//        Each frame is its own launch sequence; frames are dealt round-robin
-//        to `lanes` streams (own scratch buffers each) forked from and joined
-//        back into the caller's stream, so several frames are in flight --
-//        the way the reference overlaps frames across its per-stage threads.
+//        Groups of `per` consecutive frames (1, or 2 when the autotuned
+//        backprojector tile reconstructs two slices per workgroup) are dealt
+//        round-robin to `lanes` streams (own scratch buffers each) forked from
+//        and joined back into the caller's stream, so several frames are in
+//        flight -- the way the reference overlaps frames across its per-stage
+//        threads.
 // MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA writing
 //        the FP16 sinogram in the 4-slice interleaved layout, FP16-sinogram /
 //        FP32-accumulate backprojection.
@@ -23,7 +25,8 @@ int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows,
 
 namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
-constexpr int kPlainLanes = 4;            // frames in flight in PLAIN mode
+constexpr int kPlainLanes = 4;            // streams in PLAIN mode
+constexpr int kPlainMaxPer = 2;           // frames per lane step in PLAIN mode, at most
 }  // namespace
 
 // Graph executables a chain keeps (one per buffer set and call shape); the
@@ -40,12 +43,13 @@ struct risa_chain_s {
     int mode = 0;
     int max_frames = 0;
     int lanes = 1;
+    int per = 1;   // PLAIN: frames per lane step
     risa_geom g{};
     risa_attenuation_t att = nullptr;
     risa_fan2para_t f2p = nullptr;
     risa_filter_t filt = nullptr;
     risa_bp_t bp = nullptr;
-    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;   // PLAIN: one slot per lane
+    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;   // PLAIN: kPlainMaxPer slots per lane
     risa::DeviceBuffer<uint16_t> f16_buf;
     std::vector<hipStream_t> lane_streams;
     std::vector<hipEvent_t> lane_events;   // [0] fork, [1..lanes] joins
@@ -111,9 +115,10 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
             const size_t par = (size_t)g->numberOfParallelDetectors * g->numberOfParallelProjections;
             if (mode == RISA_CHAIN_PLAIN) {
                 h->lanes = std::min(kPlainLanes, max_frames);
-                h->att_buf.resize(fan * h->lanes);
-                h->par_buf.resize(par * h->lanes);
-                h->filt_buf.resize(par * h->lanes);
+                const size_t slots = (size_t)h->lanes * kPlainMaxPer;
+                h->att_buf.resize(fan * slots);
+                h->par_buf.resize(par * slots);
+                h->filt_buf.resize(par * slots);
                 for (int k = 0; k < h->lanes; k++) {
                     hipStream_t s;
                     RISA_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -130,11 +135,20 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
                 h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
             // backprojector tile shape, timed at the frames the chain keeps
-            // in flight (plain: one slice per lane) or at its batch (mixed);
-            // timing the chain itself per variant, eagerly, proved too noisy
-            // to rank the closest shapes
-            if (mode == RISA_CHAIN_PLAIN) ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
-            else ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
+            // in flight (plain: kPlainMaxPer slices per lane, two-slice tiles
+            // included; when a one-slice tile wins, one slice per lane) or at
+            // its batch (mixed); timing the chain itself per variant, eagerly,
+            // proved too noisy to rank the closest shapes
+            if (mode == RISA_CHAIN_PLAIN) {
+                ok(risa_hip_bp_autotune_batch(h->bp, h->lanes * kPlainMaxPer, nullptr));
+                ok(risa_hip_bp_slices_per_tile(h->bp, &h->per));
+                if (h->per < 2 || max_frames < 2 * h->lanes) {
+                    h->per = 1;
+                    ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
+                }
+            } else {
+                ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
+            }
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
             delete h;
@@ -151,19 +165,28 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
     const size_t par = (size_t)g.numberOfParallelDetectors * g.numberOfParallelProjections;
     const size_t out = (size_t)g.numberOfPixels * g.numberOfPixels;
     if (h->mode == RISA_CHAIN_PLAIN) {
-        const int lanes = std::min(h->lanes, frames);
+        const int per = h->per;
+        const int groups = (frames + per - 1) / per;
+        const int lanes = std::min(h->lanes, groups);
         RISA_HIP_CHECK(hipEventRecord(h->lane_events[0], s));
         for (int k = 0; k < lanes; k++) RISA_HIP_CHECK(hipStreamWaitEvent(h->lane_streams[k], h->lane_events[0], 0));
-        for (int f = 0; f < frames; f++) {
-            const int k = f % lanes;
+        for (int q = 0; q < groups; q++) {
+            const int k = q % lanes, f = q * per, n = std::min(per, frames - f);
             hipStream_t ls = h->lane_streams[k];
-            float* att = h->att_buf.ptr + k * fan;
-            float* pb = h->par_buf.ptr + k * par;
-            float* fb = h->filt_buf.ptr + k * par;
-            ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], att, ls));
-            ok(risa_hip_fan2para_run(h->f2p, att, planes[f], pb, ls));
-            ok(risa_filter_run_oop(h->filt, pb, fb, g.numberOfParallelProjections, ls));
-            ok(risa_hip_bp_run(h->bp, fb, img + f * out, ls));
+            float* att = h->att_buf.ptr + (size_t)k * kPlainMaxPer * fan;
+            float* pb = h->par_buf.ptr + (size_t)k * kPlainMaxPer * par;
+            float* fb = h->filt_buf.ptr + (size_t)k * kPlainMaxPer * par;
+            if (n == 1) {
+                ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], att, ls));
+                ok(risa_hip_fan2para_run(h->f2p, att, planes[f], pb, ls));
+                ok(risa_filter_run_oop(h->filt, pb, fb, g.numberOfParallelProjections, ls));
+                ok(risa_hip_bp_run(h->bp, fb, img + f * out, ls));
+            } else {
+                ok(risa_hip_attenuation_run_batch(h->att, raw + f * fan, planes + f, n, att, ls));
+                ok(risa_hip_fan2para_run_batch(h->f2p, att, planes + f, n, pb, ls));
+                ok(risa_filter_run_oop(h->filt, pb, fb, (long)n * g.numberOfParallelProjections, ls));
+                ok(risa_hip_bp_run_batch(h->bp, fb, n, img + f * out, ls));
+            }
         }
         for (int k = 0; k < lanes; k++) {
             RISA_HIP_CHECK(hipEventRecord(h->lane_events[k + 1], h->lane_streams[k]));

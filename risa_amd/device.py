@@ -209,7 +209,7 @@ def bp_kernel_name(f16: bool, variant: int) -> str:
     tx, ty, ppt, e, u, g, _ = bp_variant_shape(f16, variant)
     if f16:
         return f"risa::bp_linear_f16_kernel<{tx}, {ty}, {ppt}, {e}, {u}, {g}>"
-    return f"risa::bp_linear_kernel<{tx}, {ty}, {ppt}, {e}, {u}>"
+    return f"risa::bp_linear_kernel<{tx}, {ty}, {ppt}, {e}, {u}, {g}>"
 
 
 class Backprojection(_Handle):
@@ -259,6 +259,16 @@ class Backprojection(_Handle):
 
     def autotune_slices(self, slices: int):
         check(load().risa_hip_bp_autotune_slices(self.h, slices, _stream()))
+
+    def autotune_batch(self, slices: int):
+        """FP32 tile shape for batched launches of `slices` slices (two-slice tiles included)."""
+        check(load().risa_hip_bp_autotune_batch(self.h, slices, _stream()))
+
+    @property
+    def slices_per_tile(self) -> int:
+        n = C.c_int()
+        check(load().risa_hip_bp_slices_per_tile(self.h, C.byref(n)))
+        return n.value
 
     def autotune_f16(self, slices: int):
         check(load().risa_hip_bp_autotune_f16(self.h, slices, _stream()))

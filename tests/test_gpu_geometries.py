@@ -289,7 +289,35 @@ def test_forward_weights_match_rounding_model(risa, geom_c0, cuda, N, P, D, rot,
             err = np.abs(np.where(inside, got[i] - want, 0.0)).max()
             worst = max(worst, err / norm)
             assert err <= np.spacing(norm), f"variant {v} ({TX}x{TY}) angle {a} bin {kk}: max |diff| {err:.3e}"
-    print(f"  worst |GPU - model| over 10 variants x {S} angles: {worst:.2e} of norm", flush=True)
+    print(f"  worst |GPU - model| over {device.bp_num_variants()} variants x {S} angles: {worst:.2e} of norm",
+          flush=True)
+    bp.close()
+
+
+def test_batch_equals_single_every_variant(risa, geom_c0, cuda):
+    """Every FP32 tile variant, two-slices-per-workgroup ones included,
+    reconstructs each slice of an odd batch (the last workgroup row holds one
+    slice) bit for bit as a one-slice launch does."""
+    import torch
+    from risa_amd import device
+
+    g = geom_c0.replace(numberOfPixels=200, numberOfParallelProjections=120, numberOfParallelDetectors=160)
+    N, P, D = g.numberOfPixels, g.numberOfParallelProjections, g.numberOfParallelDetectors
+    S = 5
+    rng = np.random.default_rng(5)
+    sino = _t(rng.standard_normal((S, P, D)).astype(np.float32), cuda)
+    bp = device.Backprojection(g)
+    for v in range(device.bp_num_variants()):
+        shape = device.bp_variant_shape(False, v)
+        print(f"case: batch of {S} vs single, variant {v} {shape}", flush=True)
+        bp.variant = v
+        img = torch.full((S, N, N), float("nan"), dtype=torch.float32, device=cuda)
+        bp.batch(sino, S, img)
+        one = torch.full((N, N), float("nan"), dtype=torch.float32, device=cuda)
+        for i in range(S):
+            bp(sino[i], one)
+            _sync()
+            assert torch.equal(img[i], one), f"variant {v} {shape} slice {i}"
     bp.close()
 
 
@@ -478,8 +506,9 @@ def test_c3_filtered_uneven_geometries(risa, orc, geom_c0, cuda):
 
 
 def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
-    """The run calls that used scratch buffers (GEMM-mode in-place filter, tap
-    gradient, FP16 plain-layout backprojection, adjoint) record into a
+    """The run calls that use scratch buffers (GEMM-mode in-place filter, tap
+    gradient, FP16 plain-layout backprojection, the gather adjoint (D > N) and
+    the sheared-path adjoint in two 8-slice passes (D <= N)) record into a
     hipGraph: they allocate nothing and do not synchronise.  Each replay --
     with eager work and synchronisations between replays -- equals the eager
     result bit for bit."""
@@ -498,6 +527,10 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
     fg = device.Filter(g, mode=1)
     ff = device.Filter(g, mode=0)
     bp = device.Backprojection(g)
+    gp = g.replace(numberOfParallelDetectors=160)   # scale <= 1: sheared paths
+    bpp = device.Backprojection(gp)
+    SA = 10
+    imgs = _t(rng.standard_normal((SA, N, N)).astype(np.float32), cuda)
 
     def work(o):
         o["x"].copy_(x0)
@@ -505,10 +538,12 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
         ff.grad_taps(x0, gr, S, o["dt"])        # padded-length tap gradient
         bp.f16(h16, 256.0, S16, o["img16"])     # plain-layout FP16 through the staging buffer
         bp.adjoint(img0, o["adj"])              # transposed image buffer
+        bpp.adjoint_batch(imgs, SA, o["adjp"])  # padded copies, tile origins, partials
 
     def outs():
         return dict(x=torch.empty_like(x0), dt=torch.empty(D // 2 + 1, device=cuda),
-                    img16=torch.empty((S16, N, N), device=cuda), adj=torch.empty((P, D), device=cuda))
+                    img16=torch.empty((S16, N, N), device=cuda), adj=torch.empty((P, D), device=cuda),
+                    adjp=torch.empty((SA, P, 160), device=cuda))
 
     print("case: run calls under graph capture (GEMM filter, grad taps, FP16 BP, adjoint)", flush=True)
     eager = outs()
@@ -531,7 +566,7 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
         work(outs())                            # eager work between replays
         _sync()
     del graph
-    for h in (fg, ff, bp):
+    for h in (fg, ff, bp, bpp):
         h.close()
 
 
