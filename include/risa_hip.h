@@ -336,7 +336,8 @@ int risa_hip_bp_autotune(risa_bp_t h, void* stream);
  * counterpart.) */
 int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream);
 /* FP32 variants only, timed at `slices` slices per risa_hip_bp_run_batch
- * launch, including those that reconstruct two slices per workgroup. */
+ * launch, including those that reconstruct two slices per workgroup
+ * (when `slices` > 1). */
 int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream);
 /* Slices per workgroup of the handle's FP32 variant (1 or 2): a caller
  * batching frames gains from launching that many at a time. */
@@ -391,7 +392,7 @@ int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float*
 int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
                        float* img, int use_graph, void* stream);
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
- * 2 filtered (PLAIN: FP32, two frame slots per lane, frame 0 of a run in
+ * 2 filtered (PLAIN: FP32, four frame slots per lane, frame 0 of a run in
  * slot 0; MIXED: FP16 4-slice layout, one slot per frame). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces

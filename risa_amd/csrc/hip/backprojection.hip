@@ -169,17 +169,39 @@ __device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, flo
 // One tile of SB slices (1 or 2), shared by the per-slice kernels and the
 // persistent uneven-batch kernel.  All threads of the workgroup call it
 // together.  in_f16: the sinogram is FP16 (value * in_scale; SB = 1 only).
-// With SB = 2 the workgroup reconstructs the same tile of two slices of one
-// geometry (sino_v / I and sino2 / I2; sino2 null: only the first): a staged
-// entry holds both slices' pairs (e0, d, e0', d'), so one 16-byte LDS read
-// and one coordinate, floor and address serve two updates, and each slice's
-// arithmetic is exactly the one-slice tile's.
+// With SB > 1 the workgroup reconstructs the same tile of the nv <= SB
+// slices sino_v + b * sstride -> I + b * istride (b < nv) of one geometry: a
+// staged entry holds every slice's pair (e0, d), so one 8 * SB-byte LDS
+// read and one coordinate, floor and address serve SB updates, and each
+// slice's arithmetic is exactly the one-slice tile's.
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
+template <int SB>
+struct LdsEnt {
+    float v[2 * SB];
+};
+template <int SB>
+__device__ __forceinline__ LdsEnt<SB> lds_entry(unsigned addr)
+{
+    LdsEnt<SB> e;
+    if constexpr (SB == 1) {
+        const lds_f2 q = lds_load<lds_f2>(addr);
+        e.v[0] = q[0];
+        e.v[1] = q[1];
+    } else {
+#pragma unroll
+        for (int i = 0; i < SB / 2; i++) {
+            const lds_f4 q = lds_load<lds_f4>(addr + 16u * i);
+#pragma unroll
+            for (int j = 0; j < 4; j++) e.v[4 * i + j] = q[j];
+        }
+    }
+    return e;
+}
 template <int TX, int TY, int PPT, int E, int U, int SB = 1>
 __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* __restrict__ I,
                                         const float* __restrict__ cosT, const float* __restrict__ sinT,
                                         const BpParams& p, int x0, int y0, int in_f16, char* smem,
-                                        const float* __restrict__ sino2 = nullptr, float* __restrict__ I2 = nullptr)
+                                        long sstride = 0, long istride = 0, int nv = 1)
 {
     static_assert(SB == 1 || SB == 2, "one or two slices per tile");
     constexpr int NT = TX * TY / PPT;
@@ -189,7 +211,6 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     const int tx = tid % TX, ty = tid / TX;
     const float* S = reinterpret_cast<const float*>(sino_v);
     const uint16_t* S16 = reinterpret_cast<const uint16_t*>(sino_v);
-    const bool two = SB == 2 && sino2 != nullptr;
     const int W = p.W, AE = p.AE;
     char* rows = smem;
     const unsigned rbase = lds_address(rows);
@@ -214,8 +235,9 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
         for (int r = 0; r < PPT; r++) {
             const int y = y0 + ty + r * RS;
             if (x < p.N && y < p.N) {
-                I[(long)y * p.N + x] = p.maskingValue;
-                if (two) I2[(long)y * p.N + x] = p.maskingValue;
+#pragma unroll
+                for (int b = 0; b < SB; b++)
+                    if (b < nv) I[b * istride + (long)y * p.N + x] = p.maskingValue;
             }
         }
         return;
@@ -274,17 +296,15 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                     s0r[q][0] = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i0 * 2u));
                     s1r[q][0] = half_bits_to_float(*reinterpret_cast<const uint16_t*>(b + i1 * 2u));
                 } else {
-                    const char* b = reinterpret_cast<const char*>(S);
-                    s0r[q][0] = *reinterpret_cast<const float*>(b + i0 * 4u);
-                    s1r[q][0] = *reinterpret_cast<const float*>(b + i1 * 4u);
-                    if (SB == 2) {
-                        if (two) {
-                            const char* b2 = reinterpret_cast<const char*>(sino2);
-                            s0r[q][SB - 1] = *reinterpret_cast<const float*>(b2 + i0 * 4u);
-                            s1r[q][SB - 1] = *reinterpret_cast<const float*>(b2 + i1 * 4u);
+#pragma unroll
+                    for (int b = 0; b < SB; b++) {
+                        if (b < nv) {
+                            const char* base = reinterpret_cast<const char*>(S + b * sstride);
+                            s0r[q][b] = *reinterpret_cast<const float*>(base + i0 * 4u);
+                            s1r[q][b] = *reinterpret_cast<const float*>(base + i1 * 4u);
                         } else {
-                            s0r[q][SB - 1] = 0.0f;
-                            s1r[q][SB - 1] = 0.0f;
+                            s0r[q][b] = 0.0f;
+                            s1r[q][b] = 0.0f;
                         }
                     }
                 }
@@ -309,18 +329,21 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                     ent[2 * b] = s0 - (float)kr[q] * d;
                     ent[2 * b + 1] = d;
                 }
-                if (SB == 1)
+                if constexpr (SB == 1) {
                     reinterpret_cast<float2*>(buf)[e] = make_float2(ent[0], ent[1]);
-                else
-                    reinterpret_cast<float4*>(buf)[e] = make_float4(ent[0], ent[1], ent[2 * SB - 2], ent[2 * SB - 1]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < SB / 2; i++)
+                        reinterpret_cast<float4*>(buf)[e * (SB / 2) + i] =
+                            make_float4(ent[4 * i], ent[4 * i + 1], ent[4 * i + 2], ent[4 * i + 3]);
+                }
             }
         }
     };
-    using LdsEnt = typename std::conditional<SB == 1, lds_f2, lds_f4>::type;
     // one update of slice b from the entry q at coordinate t
-    auto update = [&](float& a, const LdsEnt& q, int b, float t) {
-        a += q[2 * b];
-        a = __builtin_fmaf(t, q[2 * b + 1], a);
+    auto update = [&](float& a, const LdsEnt<SB>& q, int b, float t) {
+        a += q.v[2 * b];
+        a = __builtin_fmaf(t, q.v[2 * b + 1], a);
     };
 
     fetch(0);
@@ -337,7 +360,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             for (; a + U <= na; a += U) {
                 float4 in[U];
                 float t[U][PPT];
-                LdsEnt v[U][PPT];
+                LdsEnt<SB> v[U][PPT];
 #pragma unroll
                 for (int u = 0; u < U; u++) in[u] = inf[a + u];
 #pragma unroll
@@ -347,7 +370,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
                     for (int r = 0; r < PPT; r++) {
                         t[u][r] = __builtin_fmaf(yp[r], in[u].y, xc);
-                        v[u][r] = lds_load<LdsEnt>(bb + (unsigned)floor_to_int(t[u][r]) * EB);
+                        v[u][r] = lds_entry<SB>(bb + (unsigned)floor_to_int(t[u][r]) * EB);
                     }
                 }
 #pragma unroll
@@ -364,7 +387,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
                 for (int r = 0; r < PPT; r++) {
                     const float t = __builtin_fmaf(yp[r], in.y, xc);
-                    const LdsEnt q = lds_load<LdsEnt>(bb + (unsigned)floor_to_int(t) * EB);
+                    const LdsEnt<SB> q = lds_entry<SB>(bb + (unsigned)floor_to_int(t) * EB);
 #pragma unroll
                     for (int b = 0; b < SB; b++) update(acc[r][b], q, b, t);
                 }
@@ -378,8 +401,10 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     for (int r = 0; r < PPT; r++) {
         const int y = y0 + ty + r * RS;
         if (x < p.N && y < p.N) {
-            I[(long)y * p.N + x] = use[r] ? acc[r][0] * p.norm * p.out_scale : p.maskingValue;
-            if (two) I2[(long)y * p.N + x] = use[r] ? acc[r][SB - 1] * p.norm * p.out_scale : p.maskingValue;
+#pragma unroll
+            for (int b = 0; b < SB; b++)
+                if (b < nv)
+                    I[b * istride + (long)y * p.N + x] = use[r] ? acc[r][b] * p.norm * p.out_scale : p.maskingValue;
         }
     }
 }
@@ -399,12 +424,11 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __
     const long sino_stride = (long)p.P * p.D, img_stride = (long)p.N * p.N;
     const float* s0 = sino + slice * sino_stride;
     float* i0 = img + slice * img_stride;
-    if (SB == 1) {
+    if constexpr (SB == 1) {
         bp_tile<TX, TY, PPT, E, U, 1>(s0, i0, cosT, sinT, p, x0, y0, 0, smem);
     } else {
-        const bool two = slice + 1 < slices;
-        bp_tile<TX, TY, PPT, E, U, SB>(s0, i0, cosT, sinT, p, x0, y0, 0, smem, two ? s0 + sino_stride : nullptr,
-                                       two ? i0 + img_stride : nullptr);
+        const int nv = (int)min((long)SB, (long)slices - slice);
+        bp_tile<TX, TY, PPT, E, U, SB>(s0, i0, cosT, sinT, p, x0, y0, 0, smem, sino_stride, img_stride, nv);
     }
 }
 
@@ -1308,7 +1332,7 @@ static const BpVariant kVariants[] = {
     // two slices per workgroup: one coordinate, floor, address and 16-byte LDS read per two updates;
     // half the staged entries per thread keep the doubled entries at ~33 KB of LDS per workgroup
     RISA_BPV(32, 32, 4, 4, 2, 2, 64), RISA_BPV(32, 32, 4, 4, 4, 2, 64), RISA_BPV(32, 16, 2, 4, 4, 2, 64),
-    RISA_BPV(16, 16, 1, 2, 4, 2, 64), RISA_BPV(16, 16, 2, 4, 2, 2, 64), RISA_BPV(32, 32, 4, 6, 2, 2, 64),
+    RISA_BPV(16, 16, 1, 2, 4, 2, 64),
 };
 #undef RISA_BPV
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -1877,7 +1901,7 @@ extern "C" int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream)
 {
     return guarded("risa_hip_bp_autotune_batch", [&] {
         if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
-        autotune(h, slices, 0, as_stream(stream), 2);
+        autotune(h, slices, 0, as_stream(stream), std::min(slices, 2));
     });
 }
 

@@ -6,12 +6,11 @@
 // stream-ordered, and a captured hipGraph replays the launch sequence.
 //
 // PLAIN: frame by frame, FP32 throughout (FFT filter, FP32 backprojector).
-//        Groups of `per` consecutive frames (1, or 2 when the autotuned
-//        backprojector tile reconstructs two slices per workgroup) are dealt
-//        round-robin to `lanes` streams (own scratch buffers each) forked from
-//        and joined back into the caller's stream, so several frames are in
-//        flight -- the way the reference overlaps frames across its per-stage
-//        threads.
+//        Groups of `per` consecutive frames (up to 4; each stage runs once
+//        per group) are dealt round-robin to `lanes` streams (own scratch
+//        buffers each) forked from and joined back into the caller's stream,
+//        so lanes x per frames are in flight -- the way the reference
+//        overlaps frames across its per-stage threads.
 // MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA writing
 //        the FP16 sinogram in the 4-slice interleaved layout, FP16-sinogram /
 //        FP32-accumulate backprojection.
@@ -26,7 +25,7 @@ int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows,
 namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
 constexpr int kPlainLanes = 4;            // streams in PLAIN mode
-constexpr int kPlainMaxPer = 2;           // frames per lane step in PLAIN mode, at most
+constexpr int kPlainMaxPer = 4;           // frames per lane step in PLAIN mode, at most
 }  // namespace
 
 // Graph executables a chain keeps (one per buffer set and call shape); the
@@ -135,17 +134,13 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
                 h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
             }
             // backprojector tile shape, timed at the frames the chain keeps
-            // in flight (plain: kPlainMaxPer slices per lane, two-slice tiles
-            // included; when a one-slice tile wins, one slice per lane) or at
-            // its batch (mixed); timing the chain itself per variant, eagerly,
-            // proved too noisy to rank the closest shapes
+            // in flight (plain: lanes x per slices, two-slice tiles included
+            // when per > 1) or at its batch (mixed); timing the chain itself
+            // per variant, eagerly, proved too noisy to rank the closest shapes
             if (mode == RISA_CHAIN_PLAIN) {
-                ok(risa_hip_bp_autotune_batch(h->bp, h->lanes * kPlainMaxPer, nullptr));
-                ok(risa_hip_bp_slices_per_tile(h->bp, &h->per));
-                if (h->per < 2 || max_frames < 2 * h->lanes) {
-                    h->per = 1;
-                    ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
-                }
+                h->per = std::max(1, std::min(kPlainMaxPer, max_frames / h->lanes));
+                if (h->per > 1) ok(risa_hip_bp_autotune_batch(h->bp, h->lanes * h->per, nullptr));
+                else ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
             } else {
                 ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
             }
