@@ -40,6 +40,7 @@ struct BpParams {
     float out_scale;     // multiplies sum * norm (1/in_scale for FP16 input)
     int AE;              // angles per staged chunk
     int zx, zy;          // log2 of the tile grid's width / height for the XCD-aware order, or -1
+    int xg;              // slice groups dealt to XCDs whole (bp_work), set per launch
 };
 
 // The tile of workgroup b.  Workgroups are dealt round-robin to the chip's 8
@@ -48,17 +49,54 @@ struct BpParams {
 // Z (Morton) order -- a compact block of the image -- which shrinks the union
 // of their sinogram windows, i.e. that XCD's L2 footprint and the HBM
 // re-reads; other grids keep the row-major order.
-__device__ __forceinline__ int2 bp_tile_xy(int b, const BpParams& p)
+__device__ __forceinline__ int2 bp_tile_of_rank(int r, const BpParams& p)
 {
-    if (p.zx < 0) return make_int2(b % p.tilesX, b / p.tilesX);
-    const int per = 1 << (p.zx + p.zy - 3);   // tiles per XCD (grid >= 8 tiles)
-    const int r = (b & 7) * per + (b >> 3);
+    if (p.zx < 0) return make_int2(r % p.tilesX, r / p.tilesX);
     int x = 0, y = 0, bit = 0;
     for (int i = 0; i < p.zx || i < p.zy; i++) {
         if (i < p.zx) x |= ((r >> bit++) & 1) << i;
         if (i < p.zy) y |= ((r >> bit++) & 1) << i;
     }
     return make_int2(x, y);
+}
+__device__ __forceinline__ int2 bp_tile_xy(int b, const BpParams& p)
+{
+    if (p.zx < 0) return make_int2(b % p.tilesX, b / p.tilesX);
+    const int per = 1 << (p.zx + p.zy - 3);   // tiles per XCD (grid >= 8 tiles)
+    return bp_tile_of_rank((b & 7) * per + (b >> 3), p);
+}
+
+// Tile and slice group of workgroup (blockIdx.x, blockIdx.y) of a grid of
+// T = gridDim.x tiles x G = gridDim.y slice groups.  With p.xg (T a multiple
+// of 8, and G a multiple or a divisor of 8) whole slice groups go to XCDs:
+// workgroup L = x + y T runs on XCD L % 8 as that XCD's (L / 8)-th; for
+// G >= 8 each XCD takes G / 8 groups, all their tiles; for G < 8, 8 / G XCDs
+// share a group, each a contiguous range of its tile order.  One XCD's L2
+// then holds few sinograms and each is fetched from HBM about once, where
+// splitting every slice's tiles over all 8 XCDs fetches it up to 8 times.
+// Otherwise the tile order of bp_tile_xy and group blockIdx.y.
+__device__ __forceinline__ int bp_work(const BpParams& p, int2& txy)
+{
+    if (!p.xg) {
+        txy = bp_tile_xy(blockIdx.x, p);
+        return blockIdx.y;
+    }
+    const int T = gridDim.x, G = gridDim.y;
+    const long L = blockIdx.x + (long)blockIdx.y * T;
+    const int xcd = (int)(L & 7);
+    const long k = L >> 3;
+    if (G >= 8) {
+        txy = bp_tile_of_rank((int)(k % T), p);
+        return xcd + 8 * (int)(k / T);
+    }
+    const int m = 8 / G;   // XCDs per group
+    txy = bp_tile_of_rank((xcd / G) * (T / m) + (int)k, p);
+    return xcd % G;
+}
+// The host's switch for bp_work.
+static inline int xcd_groups(long tiles, long groups)
+{
+    return tiles % 8 == 0 && groups > 0 && (groups % 8 == 0 || 8 % groups == 0) ? 1 : 0;
 }
 
 // Detector coordinate of a pixel; the same operation sequence everywhere
@@ -418,9 +456,9 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __
                                                                  BpParams p)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    const int2 txy = bp_tile_xy(blockIdx.x, p);
+    int2 txy;
+    const long slice = (long)bp_work(p, txy) * SB;
     const int x0 = txy.x * TX, y0 = txy.y * TY;
-    const long slice = (long)blockIdx.y * SB;
     const long sino_stride = (long)p.P * p.D, img_stride = (long)p.N * p.N;
     const float* s0 = sino + slice * sino_stride;
     float* i0 = img + slice * img_stride;
@@ -540,9 +578,9 @@ __global__ void __launch_bounds__(TX* TY / PPT) bp_linear_f16_kernel(const uint2
 
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
-    const int2 txy = bp_tile_xy(blockIdx.x, p);
+    int2 txy;
+    const int g0 = bp_work(p, txy) * G;
     const int x0 = txy.x * TX, y0 = txy.y * TY;
-    const int g0 = blockIdx.y * G;
     const int ngroups = (slices + kF16Group - 1) / kF16Group;
     const int ng = min(G, ngroups - g0);
     const int b0 = g0 * kF16Group;
@@ -1607,6 +1645,7 @@ static void launch_linear(risa_bp_t h, int variant, const float* sino, float* im
     const size_t lds = tile_lds(p.AE, p.W, 8 * v.sb);
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.sb));
+    p.xg = xcd_groups(grid.x, grid.y);
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_kernel");
 }
@@ -1651,6 +1690,7 @@ static void launch_f16x4(risa_bp_t h, int variant, const uint2* sino, float in_s
     const size_t lds = (size_t)v.g * 2 * v.e * threads * sizeof(HRow) + (size_t)3 * p.AE * 24;
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(ceil_div(slices, kF16Group), v.g));
+    p.xg = xcd_groups(grid.x, grid.y);
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_f16_kernel");
 }

@@ -294,17 +294,19 @@ def test_forward_weights_match_rounding_model(risa, geom_c0, cuda, N, P, D, rot,
     bp.close()
 
 
-def test_batch_equals_single_every_variant(risa, geom_c0, cuda):
+@pytest.mark.parametrize("N,S", [(200, 5), (256, 16), (256, 4), (256, 3)])
+def test_batch_equals_single_every_variant(risa, geom_c0, cuda, N, S):
     """Every FP32 tile variant, two-slices-per-workgroup ones included,
-    reconstructs each slice of an odd batch (the last workgroup row holds one
-    slice) bit for bit as a one-slice launch does."""
+    reconstructs each slice of a batch bit for bit as a one-slice launch
+    does: odd batches (the last workgroup row holds one slice), and batches
+    whose slice groups are dealt to XCDs whole (1, 2, 8 or 16 groups on a
+    tile grid that is a multiple of 8) or not (3 or 5 groups)."""
     import torch
     from risa_amd import device
 
-    g = geom_c0.replace(numberOfPixels=200, numberOfParallelProjections=120, numberOfParallelDetectors=160)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=120, numberOfParallelDetectors=160)
     N, P, D = g.numberOfPixels, g.numberOfParallelProjections, g.numberOfParallelDetectors
-    S = 5
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(5 + S)
     sino = _t(rng.standard_normal((S, P, D)).astype(np.float32), cuda)
     bp = device.Backprojection(g)
     for v in range(device.bp_num_variants()):
