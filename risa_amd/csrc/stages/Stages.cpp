@@ -338,7 +338,8 @@ BackprojectionMixed::BackprojectionMixed(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_backProjection");
-    const int devices = device_count(), lanes = processors_per_device(configFile);
+    const int devices = device_count();
+    const int lanes = std::max(1, config_int_or(configFile, "backProjectionLanes", processors_per_device(configFile)));
     // frames per kernel launch: the FP16 backprojector shares each pixel's
     // weights across the 4 slices of its sample layout, so it takes up to
     // `backProjectionBatch` queued frames at once (optional key, default 4)
@@ -403,28 +404,68 @@ Backprojection::Backprojection(const std::string& configFile)
 {
     g_ = read_geometry(configFile);
     const int pool = config_int(configFile, "memPoolSize_backProjection");
-    const int devices = device_count(), lanes = processors_per_device(configFile);
+    // processor threads: the optional key backProjectionLanes, default
+    // processorsPerDevice
+    const int devices = device_count();
+    const int lanes = std::max(1, config_int_or(configFile, "backProjectionLanes", processors_per_device(configFile)));
+    // frames per kernel launch: up to `backProjectionBatch` queued frames
+    // (optional key, default 4) go to one batched launch, whose tiles may
+    // reconstruct two slices per workgroup; the tile shape is timed at the
+    // frames the device's lanes then keep in flight
+    batch_ = std::max(1, std::min(config_int_or(configFile, "backProjectionBatch", 4), std::max(1, pool)));
+    const std::size_t plane = (std::size_t)g_.numberOfParallelProjections * g_.numberOfParallelDetectors;
     for (int w = 0; w < devices * lanes; w++) {
         risa_bp_t h = nullptr;
         check(risa_hip_bp_create(&g_, w / lanes, &h), "Backprojection");
         h_.push_back(h);
         glados::hip::ScopedDevice sd(w / lanes);
-        // tile shape for the number of frames the device's lanes keep in flight
         int variant = 0;
-        if (w % lanes == 0) check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
+        if (w % lanes == 0) {
+            if (batch_ > 1) check(risa_hip_bp_autotune_batch(h, lanes * batch_, nullptr), "Backprojection autotune");
+            else check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
+        }
         check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "Backprojection variant");
         check(risa_hip_bp_set_variant(h, variant), "Backprojection variant");
+        // per processor: contiguous staging for a batch of sinograms and images
+        float* in32 = nullptr;
+        float* out32 = nullptr;
+        if (batch_ > 1) {
+            hip_check(hipMalloc(&in32, (std::size_t)batch_ * plane * sizeof(float)), "Backprojection staging");
+            hip_check(hipMalloc(&out32, (std::size_t)batch_ * img_elems(g_) * sizeof(float)), "Backprojection staging");
+        }
+        in32_.push_back(in32);
+        out32_.push_back(out32);
     }
-    start("Backprojection", devices, lanes, pool, img_elems(g_),
-          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
-              check(risa_hip_bp_run(h_[(std::size_t)w], in.data(), out.data(), s), "Backprojection::processor");
-          });
+    start_batched("Backprojection", devices, lanes, pool, img_elems(g_), batch_,
+                  [this, plane](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      const risa_bp_t h = h_[(std::size_t)w];
+                      if (n == 1) {
+                          check(risa_hip_bp_run(h, ins[0].data(), outs[0].data(), s), "Backprojection::processor");
+                          return;
+                      }
+                      float* in32 = in32_[(std::size_t)w];
+                      float* out32 = out32_[(std::size_t)w];
+                      const std::size_t img = img_elems(g_);
+                      for (int i = 0; i < n; i++)
+                          hip_check(hipMemcpyAsync(in32 + (std::size_t)i * plane, ins[i].data(), plane * sizeof(float),
+                                                   hipMemcpyDeviceToDevice, s),
+                                    "Backprojection gather");
+                      check(risa_hip_bp_run_batch(h, in32, n, out32, s), "Backprojection::processor");
+                      for (int i = 0; i < n; i++)
+                          hip_check(hipMemcpyAsync(outs[i].data(), out32 + (std::size_t)i * img, img * sizeof(float),
+                                                   hipMemcpyDeviceToDevice, s),
+                                    "Backprojection scatter");
+                  });
 }
 
 Backprojection::~Backprojection()
 {
     stop();
     for (auto h : h_) risa_hip_bp_destroy(h);
+    for (auto p : in32_)
+        if (p) (void)hipFree(p);
+    for (auto p : out32_)
+        if (p) (void)hipFree(p);
 }
 
 // -------------------------------------------------------------- Masking --
