@@ -35,7 +35,7 @@ METRIC = "512² slices/sec on 1 GPU (mixed vs plain FP32) at stated RMSE; BP HBM
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_UPDATE = 10          # t: mul+fma; two weights; two accumulate FMAs
-BP_SLICES_IN_FLIGHT = 16      # the plain chain keeps 4 lanes x 4 frames in flight
+BP_SLICES_IN_FLIGHT = 32      # the plain chain keeps 4 lanes x 8 frames in flight
 
 
 def log(*a):
@@ -219,7 +219,7 @@ def main():
                     help="time one kernel alone (for rocprofv3 runs) and exit")
     ap.add_argument("--launches", type=int, default=50, help="launches in --kernel mode")
     ap.add_argument("--variant", type=int, default=None, help="--kernel bp/bp16: this tile variant, not the autotuned one")
-    ap.add_argument("--slices", type=int, default=None, help="--kernel bp / bp16: slices per launch (default 16 / 32)")
+    ap.add_argument("--slices", type=int, default=None, help="--kernel bp / bp16: slices per launch (default 32 / 32)")
     ap.add_argument("--adjoint-gather", action="store_true", help="--kernel adj: the gather adjoint kernel")
     args = ap.parse_args()
 
@@ -297,8 +297,8 @@ def main():
     img_mixed_sample = img[:planes].cpu().numpy()
 
     # Dominant kernel: backprojection, timed with HIP events on its stream,
-    # at the concurrency the plain chain runs it (4 lanes x 4 frames in
-    # flight = one launch of 16 slices here, tile shape autotuned for it).
+    # at the concurrency the plain chain runs it (4 lanes x 8 frames in
+    # flight = one launch of 32 slices here, tile shape autotuned for it).
     P, D = g.numberOfParallelProjections, g.numberOfParallelDetectors
     S = BP_SLICES_IN_FLIGHT
     bp = device.Backprojection(g)

@@ -322,8 +322,8 @@ int risa_hip_bp_set_variant(risa_bp_t h, int variant);
 int risa_hip_bp_get_variant(risa_bp_t h, int* variant);
 /* Tile shape of a variant (f16 = 0: FP32 kernel, 1: FP16 kernel):
  * shape = {tile x, tile y, pixels per thread, staged entries per thread,
- * angles per step, slices per workgroup (FP16: groups of 4 slices; FP32: 1 or
- * 2, two slices of a batch sharing each pixel's coordinate), max angles per chunk};
+ * angles per step, slices per workgroup (FP16: groups of 4 slices; FP32: 1, 2
+ * or 4 slices of a batch sharing each pixel's coordinate), max angles per chunk};
  * RISA_ERR_ARG past the last variant.  (Tooling; no reference counterpart.) */
 int risa_hip_bp_variant_shape(int f16, int variant, int shape[7]);
 /* Times every variant that fits the geometry and keeps the fastest: FP32
@@ -336,10 +336,10 @@ int risa_hip_bp_autotune(risa_bp_t h, void* stream);
  * counterpart.) */
 int risa_hip_bp_autotune_slices(risa_bp_t h, int slices, void* stream);
 /* FP32 variants only, timed at `slices` slices per risa_hip_bp_run_batch
- * launch, including those that reconstruct two slices per workgroup
- * (when `slices` > 1). */
+ * launch, including those that reconstruct two or four slices per
+ * workgroup (at most `slices`). */
 int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream);
-/* Slices per workgroup of the handle's FP32 variant (1 or 2): a caller
+/* Slices per workgroup of the handle's FP32 variant (1, 2 or 4): a caller
  * batching frames gains from launching that many at a time. */
 int risa_hip_bp_slices_per_tile(risa_bp_t h, int* slices);
 /* FP16 variants only, timed at `slices` slices per risa_hip_bp_run_f16x4
@@ -392,7 +392,7 @@ int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float*
 int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
                        float* img, int use_graph, void* stream);
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
- * 2 filtered (PLAIN: FP32, four frame slots per lane, frame 0 of a run in
+ * 2 filtered (PLAIN: FP32, eight frame slots per lane, frame 0 of a run in
  * slot 0; MIXED: FP16 4-slice layout, one slot per frame). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces

@@ -204,7 +204,7 @@ __device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, flo
 // (zeros outside [0, D)), so that s[k] + (t' - k') d = e0 + t' d and the
 // update needs no fraction: acc += e0; acc = fma(t', d, acc).
 //
-// One tile of SB slices (1 or 2), shared by the per-slice kernels and the
+// One tile of SB slices (1, 2 or 4), shared by the per-slice kernels and the
 // persistent uneven-batch kernel.  All threads of the workgroup call it
 // together.  in_f16: the sinogram is FP16 (value * in_scale; SB = 1 only).
 // With SB > 1 the workgroup reconstructs the same tile of the nv <= SB
@@ -213,6 +213,14 @@ __device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, flo
 // read and one coordinate, floor and address serve SB updates, and each
 // slice's arithmetic is exactly the one-slice tile's.
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
+// Staged row buffers of a tile of SB slices per workgroup.
+__host__ __device__ constexpr int bp_row_buffers(int sb) { return sb == 4 ? 1 : 2; }
+// Four-slice entries are two 16-byte halves in two planes kQuadPlane bytes
+// apart (slices 0-1, slices 2-3), so a wave's reads of either half are
+// 16-byte strided like the two-slice tiles' (a 32-byte entry would put
+// adjacent lanes' 16-byte reads 32 bytes apart) and both halves share one
+// address, the second read taking the plane offset as its immediate.
+constexpr unsigned kQuadPlane = 16384;
 template <int SB>
 struct LdsEnt {
     float v[2 * SB];
@@ -228,7 +236,7 @@ __device__ __forceinline__ LdsEnt<SB> lds_entry(unsigned addr)
     } else {
 #pragma unroll
         for (int i = 0; i < SB / 2; i++) {
-            const lds_f4 q = lds_load<lds_f4>(addr + 16u * i);
+            const lds_f4 q = lds_load<lds_f4>(addr + kQuadPlane * i);
 #pragma unroll
             for (int j = 0; j < 4; j++) e.v[4 * i + j] = q[j];
         }
@@ -241,10 +249,15 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                                         const BpParams& p, int x0, int y0, int in_f16, char* smem,
                                         long sstride = 0, long istride = 0, int nv = 1)
 {
-    static_assert(SB == 1 || SB == 2, "one or two slices per tile");
+    static_assert(SB == 1 || SB == 2 || SB == 4, "one, two or four slices per tile");
+    // staged row buffers: two (the next chunk is committed while the current
+    // one is read), one for four-slice tiles (their entries would otherwise
+    // leave few angles per chunk; the commit then waits for a barrier after
+    // the chunk is read)
+    constexpr int NB = bp_row_buffers(SB);
     constexpr int NT = TX * TY / PPT;
     constexpr int RS = TY / PPT;
-    constexpr unsigned EB = 8u * SB;   // bytes per staged entry
+    constexpr unsigned EB = SB == 4 ? 16u : 8u * SB;   // entry stride in a row (a plane for SB = 4)
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const float* S = reinterpret_cast<const float*>(sino_v);
@@ -252,7 +265,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     const int W = p.W, AE = p.AE;
     char* rows = smem;
     const unsigned rbase = lds_address(rows);
-    float4* info = reinterpret_cast<float4*>(smem + (size_t)2 * AE * W * EB);
+    float4* info = reinterpret_cast<float4*>(smem + (SB == 4 ? (size_t)2 * kQuadPlane : (size_t)NB * AE * W * EB));
     int2* kinfo = reinterpret_cast<int2*>(info + 3 * AE);
 
     const int x = x0 + tx;
@@ -299,7 +312,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
         int2* kin = kinfo + (ch % 3) * AE;
         for (int a = tid; a < na; a += NT) {
             const AngleWin w = angle_window(cosT[p0 + a], sinT[p0 + a], xa, xb, ya, yb, p.cI);
-            const int row = (ch & 1) * AE * W + a * W;
+            const int row = (ch % NB) * AE * W + a * W;
             inf[a] = make_float4(w.c, w.s, w.negT, __uint_as_float(rbase + (unsigned)(row - w.kmin) * EB));
             kin[a] = make_int2(w.kmin, w.kabs);
         }
@@ -354,7 +367,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
     };
     auto commit = [&](int ch) {
         const int nae = min(AE, p.P - ch * AE) * W;
-        char* buf = rows + (size_t)(ch & 1) * AE * W * EB;
+        char* buf = rows + (size_t)(ch % NB) * AE * W * EB;
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int e = tid + q * NT;
@@ -372,7 +385,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 } else {
 #pragma unroll
                     for (int i = 0; i < SB / 2; i++)
-                        reinterpret_cast<float4*>(buf)[e * (SB / 2) + i] =
+                        reinterpret_cast<float4*>(buf + kQuadPlane * i)[e] =
                             make_float4(ent[4 * i], ent[4 * i + 1], ent[4 * i + 2], ent[4 * i + 3]);
                 }
             }
@@ -431,6 +444,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
                 }
             }
         }
+        if constexpr (NB == 1) __syncthreads();   // the single row buffer is read
         if (ch + 1 < nch) commit(ch + 1);
         make_info(ch + 2);
         __syncthreads();
@@ -1371,6 +1385,9 @@ static const BpVariant kVariants[] = {
     // half the staged entries per thread keep the doubled entries at ~33 KB of LDS per workgroup
     RISA_BPV(32, 32, 4, 4, 2, 2, 64), RISA_BPV(32, 32, 4, 4, 4, 2, 64), RISA_BPV(32, 16, 2, 4, 4, 2, 64),
     RISA_BPV(16, 16, 1, 2, 4, 2, 64),
+    // four slices per workgroup: two planes of 16-byte half entries, one staged row buffer
+    RISA_BPV(32, 32, 4, 4, 1, 4, 64), RISA_BPV(32, 32, 4, 4, 2, 4, 64), RISA_BPV(32, 16, 2, 4, 2, 4, 64),
+    RISA_BPV(16, 16, 1, 2, 4, 4, 64),
 };
 #undef RISA_BPV
 static constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -1404,19 +1421,22 @@ static int window_entries(int tx, int ty, float scale)
     return (int)std::ceil(diag * (double)scale * 1.00001 + 1e-2) + 2;
 }
 
-// Angles per chunk and the dynamic LDS bytes of a tile:
-// 2 row buffers x AE x W entries of entry_bytes + a ring of 3 chunks of
-// per-angle windows (16 B + 8 B per angle).
+// Angles per chunk and the dynamic LDS bytes of a tile of sb slices per
+// workgroup: 2 row buffers x AE x W entries of 8 * sb bytes (sb = 4: one
+// buffer of two kQuadPlane-byte planes of 16-byte half entries) + a ring of
+// 3 chunks of per-angle windows (16 B + 8 B per angle).
 constexpr size_t kMaxLds = 64 * 1024;
-static size_t tile_lds(int AE, int W, int entry_bytes)
+static size_t tile_lds(int AE, int W, int sb)
 {
-    return (size_t)2 * AE * W * entry_bytes + (size_t)3 * AE * 24;
+    const size_t rows = sb == 4 ? (size_t)2 * kQuadPlane : (size_t)2 * AE * W * 8 * sb;
+    return rows + (size_t)3 * AE * 24;
 }
-// AE also shrinks so the tile fits the LDS budget.
-static int chunk_angles(int threads, int e, int amax, int W, int entry_bytes = 8)
+// AE also shrinks so the tile fits the LDS budget (sb = 4: a plane).
+static int chunk_angles(int threads, int e, int amax, int W, int sb = 1)
 {
     int ae = std::min(amax, e * threads / W);
-    ae = std::min<long>(ae, (long)kMaxLds / (2L * W * entry_bytes + 72));
+    if (sb == 4) ae = std::min<long>(ae, (long)kQuadPlane / (16L * W));
+    else ae = std::min<long>(ae, (long)kMaxLds / (2L * W * 8 * sb + 72));
     return std::max(1, ae);
 }
 
@@ -1641,8 +1661,8 @@ static void launch_linear(risa_bp_t h, int variant, const float* sino, float* im
     const BpVariant& v = kVariants[variant];
     BpParams p = make_params(h, v.tx, v.ty);
     const int threads = v.tx * v.ty / v.ppt;
-    p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8 * v.sb);
-    const size_t lds = tile_lds(p.AE, p.W, 8 * v.sb);
+    p.AE = chunk_angles(threads, v.e, v.amax, p.W, v.sb);
+    const size_t lds = tile_lds(p.AE, p.W, v.sb);
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.sb));
     p.xg = xcd_groups(grid.x, grid.y);
@@ -1941,7 +1961,7 @@ extern "C" int risa_hip_bp_autotune_batch(risa_bp_t h, int slices, void* stream)
 {
     return guarded("risa_hip_bp_autotune_batch", [&] {
         if (!h || slices < 1) fail(RISA_ERR_ARG, "bad argument");
-        autotune(h, slices, 0, as_stream(stream), std::min(slices, 2));
+        autotune(h, slices, 0, as_stream(stream), std::min(slices, 4));
     });
 }
 
@@ -2085,8 +2105,8 @@ static void batched_layout(risa_bp_batched_s* h)
         const int tilesY = ceil_div(j.N, v.ty);
         p.invW = 1.0f / (float)p.W;
         p.out_scale = j.dtype == RISA_F16 ? 1.0f / j.in_scale : 1.0f;
-        p.AE = chunk_angles(threads, v.e, v.amax, p.W, 8);
-        lds = std::max(lds, tile_lds(p.AE, p.W, sizeof(float2)));
+        p.AE = chunk_angles(threads, v.e, v.amax, p.W, 1);
+        lds = std::max(lds, tile_lds(p.AE, p.W, 1));
         dj[i].sino = j.sino;
         dj[i].img = j.img;
         dj[i].lut_off = h->lut_off[i];

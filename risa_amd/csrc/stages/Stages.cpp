@@ -421,7 +421,11 @@ Backprojection::Backprojection(const std::string& configFile)
         glados::hip::ScopedDevice sd(w / lanes);
         int variant = 0;
         if (w % lanes == 0) {
-            if (batch_ > 1) check(risa_hip_bp_autotune_batch(h, lanes * batch_, nullptr), "Backprojection autotune");
+            // the optional key backProjectionVariant fixes the tile shape
+            // (risa_hip_bp_variant_shape's numbering) instead of timing them
+            const int fixed = config_int_or(configFile, "backProjectionVariant", -1);
+            if (fixed >= 0) check(risa_hip_bp_set_variant(h, fixed), "Backprojection variant");
+            else if (batch_ > 1) check(risa_hip_bp_autotune_batch(h, lanes * batch_, nullptr), "Backprojection autotune");
             else check(risa_hip_bp_autotune_slices(h, lanes, nullptr), "Backprojection autotune");
         }
         check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "Backprojection variant");
