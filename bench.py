@@ -604,7 +604,8 @@ def main():
                   "vs_plain": round(mixed_value / plain_value, 3), "dtype": "f16 storage / f32 accumulate"},
         "accuracy_vs_fp64_oracle": rmse,
         "roofline": {"bound": "valu", "resource": "fp32 vector ALU (backprojection is compute-bound; "
-                                                  "AI ~850 flop/B >> ridge ~20)",
+                                                  "AI ~850 flop/B >> ridge ~20; its LDS reads run beside "
+                                                  "it near their measured ceiling)",
                      "kernel": f"bp_linear_kernel variant {bp_variant}, {S} slices per launch",
                      "achieved": round(bp_tflops, 2), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(bp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
