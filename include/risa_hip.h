@@ -169,6 +169,12 @@ int risa_hip_attenuation_run(risa_attenuation_t h, const uint16_t* raw, int plan
 /* `frames` consecutive frames; planes[f] is a host array. */
 int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16_t* raw,
                                    const int32_t* planes, int frames, float* out, void* stream);
+/* The batch over separately allocated frames: frame f is raw[f] -> out[f]
+ * (host arrays of device pointers).  No reference counterpart: for callers
+ * that batch queued frames without copying them together (the stage layer). */
+int risa_hip_attenuation_run_list(risa_attenuation_t h, const uint16_t* const* raw,
+                                  const int32_t* planes, int frames, float* const* out,
+                                  void* stream);
 int risa_hip_attenuation_destroy(risa_attenuation_t h);
 /* Input layout of the raw frames the handle reads (default DETECTOR_MAJOR).
  * MODULE_MAJOR fuses the Reordering stage into the attenuation load: the
@@ -238,6 +244,11 @@ int risa_hip_fan2para_run(risa_fan2para_t h, const float* fan, int plane, float*
                           void* stream);
 int risa_hip_fan2para_run_batch(risa_fan2para_t h, const float* fan, const int32_t* planes,
                                 int frames, float* par, void* stream);
+/* The batch over separately allocated frames: frame f is fan[f] -> par[f]
+ * (host arrays of device pointers).  No reference counterpart: for callers
+ * that batch queued frames without copying them together (the stage layer). */
+int risa_hip_fan2para_run_list(risa_fan2para_t h, const float* const* fan, const int32_t* planes,
+                               int frames, float* const* par, void* stream);
 /* Device bytes of the packed table (all planes). */
 int risa_hip_fan2para_table_bytes(risa_fan2para_t h, size_t* bytes);
 int risa_hip_fan2para_destroy(risa_fan2para_t h);
@@ -251,6 +262,11 @@ int risa_hip_filter_set_taps(risa_filter_t h, const float* taps);
 int risa_hip_filter_get_taps(risa_filter_t h, float* taps);
 /* In place, `slices` sinograms [P][D] (Filter::processor, Filter.cu:102-143). */
 int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, void* stream);
+/* In place over separately allocated sinograms sino[0 .. slices-1] (a host
+ * array of device pointers), FFT32 mode (RISA_ERR_UNSUPPORTED otherwise).
+ * No reference counterpart: for callers that batch queued frames without
+ * copying them together (the stage layer). */
+int risa_hip_filter_run_list(risa_filter_t h, float* const* sino, int slices, void* stream);
 /* Out of place with FP16 output (value * out_scale stored as half). */
 int risa_hip_filter_run_f16(risa_filter_t h, const float* sino, uint16_t* out, int slices,
                             float out_scale, void* stream);
@@ -294,6 +310,11 @@ int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void* stream);
  * as a grid dimension (no reference counterpart; the reference launches one
  * slice per processor call, Backprojection.cu:190-200). */
 int risa_hip_bp_run_batch(risa_bp_t h, const float* sino, int slices, float* img, void* stream);
+/* The batch over separately allocated slices: sino[k] -> img[k] (host arrays
+ * of device pointers).  No reference counterpart: for callers that batch
+ * queued frames without copying them together (the stage layer). */
+int risa_hip_bp_run_list(risa_bp_t h, const float* const* sino, int slices, float* const* img,
+                         void* stream);
 /* `slices` sinograms of this geometry stored as FP16 (value * in_scale),
  * FP32 accumulation; img [slices][N][N].  Linear interpolation only. */
 int risa_hip_bp_run_f16(risa_bp_t h, const uint16_t* sino, float in_scale, int slices,

@@ -151,6 +151,7 @@ _SIGS = {
     "risa_hip_attenuation_create": [C.POINTER(Geom), P, P, C.c_int, C.POINTER(P)],
     "risa_hip_attenuation_run": [P, P, C.c_int, P, P],
     "risa_hip_attenuation_run_batch": [P, P, P, C.c_int, P, P],
+    "risa_hip_attenuation_run_list": [P, P, P, C.c_int, P, P],
     "risa_hip_attenuation_destroy": [P],
     "risa_hip_attenuation_set_input_layout": [P, C.c_int],
     "risa_hip_attenuation_get_input_layout": [P, C.POINTER(C.c_int)],
@@ -163,12 +164,14 @@ _SIGS = {
     "risa_hip_fan2para_create_ex": [C.POINTER(Geom), C.c_int, C.c_int, C.POINTER(P)],
     "risa_hip_fan2para_run": [P, P, C.c_int, P, P],
     "risa_hip_fan2para_run_batch": [P, P, P, C.c_int, P, P],
+    "risa_hip_fan2para_run_list": [P, P, P, C.c_int, P, P],
     "risa_hip_fan2para_table_bytes": [P, C.POINTER(C.c_size_t)],
     "risa_hip_fan2para_destroy": [P],
     "risa_hip_filter_create": [C.POINTER(Geom), C.c_int, C.c_int, C.POINTER(P)],
     "risa_hip_filter_set_taps": [P, P],
     "risa_hip_filter_get_taps": [P, P],
     "risa_hip_filter_run": [P, P, C.c_int, P],
+    "risa_hip_filter_run_list": [P, P, C.c_int, P],
     "risa_hip_filter_run_f16": [P, P, P, C.c_int, C.c_float, P],
     "risa_hip_filter_run_f16x4": [P, P, P, C.c_int, C.c_float, P],
     "risa_hip_filter_adjoint_run": [P, P, C.c_int, P],
@@ -178,6 +181,7 @@ _SIGS = {
     "risa_hip_bp_create": [C.POINTER(Geom), C.c_int, C.POINTER(P)],
     "risa_hip_bp_run": [P, P, P, P],
     "risa_hip_bp_run_batch": [P, P, C.c_int, P, P],
+    "risa_hip_bp_run_list": [P, P, C.c_int, P, P],
     "risa_hip_bp_run_f16": [P, P, C.c_float, C.c_int, P, P],
     "risa_hip_bp_run_f16x4": [P, P, C.c_float, C.c_int, P, P],
     "risa_hip_bp_adjoint_run": [P, P, P, P],

@@ -24,8 +24,13 @@
 
 namespace risa {
 
-struct PlaneList {
-    int32_t p[64];
+// The frames of one launch: frame f reads raw[f], writes out[f] and uses
+// plane p[f]'s reference (a contiguous batch fills the pointers by stride).
+constexpr int kAttFrames = 64;
+struct AttFrames {
+    const uint16_t* raw[kAttFrames];
+    float* out[kAttFrames];
+    int32_t p[kAttFrames];
 };
 
 constexpr long kAttGrid = 256;   // workgroups per frame (grid-stride beyond)
@@ -77,19 +82,17 @@ __global__ void __launch_bounds__(256) reorder4_kernel(const uint16_t* __restric
     }
 }
 
-__global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __restrict__ raw,
-                                                          const float* __restrict__ refm,
-                                                          const float* __restrict__ dark,
-                                                          float* __restrict__ out, int Df, int quads_per_row,
-                                                          int rows, PlaneList planes, int module_major)
+__global__ void __launch_bounds__(256) attenuation_kernel(const float* __restrict__ refm,
+                                                          const float* __restrict__ dark, int Df, int quads_per_row,
+                                                          int rows, AttFrames frames, int module_major)
 {
     const int f = blockIdx.y;
-    const int plane = planes.p[f];
+    const int plane = frames.p[f];
     const long n = (long)Df * rows;
-    const uint16_t* r = raw + (long)f * n;
+    const uint16_t* __restrict__ r = frames.raw[f];
     const float* rm = refm + (long)plane * n;
     const float* dk = dark + (long)plane * Df;
-    float* o = out + (long)f * n;
+    float* __restrict__ o = frames.out[f];
     const float temp = 1.0e-5f;   // (float)pow(10,-5)
     // one iteration: 4 detectors of one row; at most kAttGrid workgroups
     // per frame stride over it
@@ -128,22 +131,20 @@ __global__ void __launch_bounds__(256) attenuation_kernel(const uint16_t* __rest
 }
 
 // Any D_f: one element per thread.
-__global__ void __launch_bounds__(256) attenuation_kernel_scalar(const uint16_t* __restrict__ raw,
-                                                                 const float* __restrict__ refm,
-                                                                 const float* __restrict__ dark,
-                                                                 float* __restrict__ out, int Df, int rows,
-                                                                 PlaneList planes, int module_major)
+__global__ void __launch_bounds__(256) attenuation_kernel_scalar(const float* __restrict__ refm,
+                                                                 const float* __restrict__ dark, int Df, int rows,
+                                                                 AttFrames frames, int module_major)
 {
     const int f = blockIdx.y;
-    const int plane = planes.p[f];
+    const int plane = frames.p[f];
     const long n = (long)Df * rows;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const int x = (int)(e % Df);
-    float num = (float)raw[(long)f * n + (module_major ? reorder_source(e, rows) : e)] - dark[(long)plane * Df + x];
+    float num = (float)frames.raw[f][module_major ? reorder_source(e, rows) : e] - dark[(long)plane * Df + x];
     if (num < 1.0e-5f) num = 1.0e-5f;
     const float den = refm[(long)plane * n + e];
-    out[(long)f * n + e] = -logf(num / fabsf(den)) * (den > 0.0f ? 1.0f : 0.0f);
+    frames.out[f][e] = -logf(num / fabsf(den)) * (den > 0.0f ? 1.0f : 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,34 +393,45 @@ extern "C" int risa_hip_attenuation_create(const risa_geom* g, const float* avg_
     });
 }
 
-static void launch_attenuation(risa_attenuation_t h, const uint16_t* raw, const int32_t* planes, int frames,
-                               float* out, hipStream_t s)
+// Frame f of the call reads raw(f) and writes out(f).
+template <class RawAt, class OutAt>
+static void launch_attenuation_at(risa_attenuation_t h, RawAt raw, const int32_t* planes, int frames, OutAt out,
+                               hipStream_t s)
 {
     const int Df = h->g.numberOfFanDetectors, Pf = h->g.numberOfFanProjections;
-    for (int f0 = 0; f0 < frames; f0 += 64) {
-        const int nf = std::min(64, frames - f0);
-        PlaneList pl{};
+    for (int f0 = 0; f0 < frames; f0 += kAttFrames) {
+        const int nf = std::min(kAttFrames, frames - f0);
+        AttFrames fr{};
+        bool vec = Df % 4 == 0;
         for (int f = 0; f < nf; f++) {
             if (planes[f0 + f] < 0 || planes[f0 + f] >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
-            pl.p[f] = planes[f0 + f];
+            fr.p[f] = planes[f0 + f];
+            fr.raw[f] = raw(f0 + f);
+            fr.out[f] = out(f0 + f);
+            if (!fr.raw[f] || !fr.out[f]) fail(RISA_ERR_ARG, "null frame pointer");
+            vec = vec && (uintptr_t)fr.raw[f] % 8 == 0 && (uintptr_t)fr.out[f] % 16 == 0;
         }
         const size_t n = (size_t)Df * Pf;
-        const uint16_t* r = raw + (size_t)f0 * n;
-        float* o = out + (size_t)f0 * n;
-        const bool vec = (Df % 4 == 0) && ((uintptr_t)r % 8 == 0) && ((uintptr_t)o % 16 == 0);
         const int mm = h->layout == RISA_LAYOUT_MODULE_MAJOR ? 1 : 0;
         if (vec) {
             const int quads = Df / 4;
             dim3 grid((unsigned)std::min<long>(ceil_div((long)quads * Pf, 256), kAttGrid), nf);
-            hipLaunchKernelGGL(attenuation_kernel, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df, quads,
-                               Pf, pl, mm);
+            hipLaunchKernelGGL(attenuation_kernel, grid, dim3(256), 0, s, h->refm.ptr, h->dark.ptr, Df, quads, Pf, fr,
+                               mm);
         } else {
             dim3 grid(ceil_div((long)n, 256), nf);
-            hipLaunchKernelGGL(attenuation_kernel_scalar, grid, dim3(256), 0, s, r, h->refm.ptr, h->dark.ptr, o, Df,
-                               Pf, pl, mm);
+            hipLaunchKernelGGL(attenuation_kernel_scalar, grid, dim3(256), 0, s, h->refm.ptr, h->dark.ptr, Df, Pf, fr,
+                               mm);
         }
         check_launch("attenuation_kernel");
     }
+}
+static void launch_attenuation(risa_attenuation_t h, const uint16_t* raw, const int32_t* planes, int frames,
+                               float* out, hipStream_t s)
+{
+    const size_t n = (size_t)h->g.numberOfFanDetectors * h->g.numberOfFanProjections;
+    launch_attenuation_at(
+        h, [&](int f) { return raw + (size_t)f * n; }, planes, frames, [&](int f) { return out + (size_t)f * n; }, s);
 }
 
 extern "C" int risa_hip_attenuation_run(risa_attenuation_t h, const uint16_t* raw, int plane, float* out,
@@ -439,6 +451,16 @@ extern "C" int risa_hip_attenuation_run_batch(risa_attenuation_t h, const uint16
         if (!h || !raw || !out || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
         if (frames == 0) return;
         launch_attenuation(h, raw, planes, frames, out, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_attenuation_run_list(risa_attenuation_t h, const uint16_t* const* raw, const int32_t* planes,
+                                             int frames, float* const* out, void* stream)
+{
+    return guarded("risa_hip_attenuation_run_list", [&] {
+        if (!h || !raw || !out || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
+        launch_attenuation_at(
+            h, [&](int f) { return raw[f]; }, planes, frames, [&](int f) { return out[f]; }, as_stream(stream));
     });
 }
 

@@ -208,7 +208,7 @@ __device__ __forceinline__ AngleWin angle_window(float c, float s, float xa, flo
 // persistent uneven-batch kernel.  All threads of the workgroup call it
 // together.  in_f16: the sinogram is FP16 (value * in_scale; SB = 1 only).
 // With SB > 1 the workgroup reconstructs the same tile of the nv <= SB
-// slices sino_v + b * sstride -> I + b * istride (b < nv) of one geometry: a
+// slices sp[b] -> ip[b] (b < nv; sp[0] = sino_v, ip[0] = I) of one geometry: a
 // staged entry holds every slice's pair (e0, d), so one 8 * SB-byte LDS
 // read and one coordinate, floor and address serve SB updates, and each
 // slice's arithmetic is exactly the one-slice tile's.
@@ -247,7 +247,7 @@ template <int TX, int TY, int PPT, int E, int U, int SB = 1>
 __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* __restrict__ I,
                                         const float* __restrict__ cosT, const float* __restrict__ sinT,
                                         const BpParams& p, int x0, int y0, int in_f16, char* smem,
-                                        long sstride = 0, long istride = 0, int nv = 1)
+                                        const float* const* sp = nullptr, float* const* ip = nullptr, int nv = 1)
 {
     static_assert(SB == 1 || SB == 2 || SB == 4, "one, two or four slices per tile");
     // staged row buffers: two (the next chunk is committed while the current
@@ -288,7 +288,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
             if (x < p.N && y < p.N) {
 #pragma unroll
                 for (int b = 0; b < SB; b++)
-                    if (b < nv) I[b * istride + (long)y * p.N + x] = p.maskingValue;
+                    if (b < nv) (SB == 1 ? I : ip[b])[(long)y * p.N + x] = p.maskingValue;
             }
         }
         return;
@@ -350,7 +350,7 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
                     for (int b = 0; b < SB; b++) {
                         if (b < nv) {
-                            const char* base = reinterpret_cast<const char*>(S + b * sstride);
+                            const char* base = reinterpret_cast<const char*>(SB == 1 ? S : sp[b]);
                             s0r[q][b] = *reinterpret_cast<const float*>(base + i0 * 4u);
                             s1r[q][b] = *reinterpret_cast<const float*>(base + i1 * 4u);
                         } else {
@@ -456,31 +456,42 @@ __device__ __forceinline__ void bp_tile(const void* __restrict__ sino_v, float* 
 #pragma unroll
             for (int b = 0; b < SB; b++)
                 if (b < nv)
-                    I[b * istride + (long)y * p.N + x] = use[r] ? acc[r][b] * p.norm * p.out_scale : p.maskingValue;
+                    (SB == 1 ? I : ip[b])[(long)y * p.N + x] =
+                        use[r] ? acc[r][b] * p.norm * p.out_scale : p.maskingValue;
         }
     }
 }
 
 // Workgroup (x, y): tile x of slices SB*y .. SB*y + SB - 1 (those below `slices`).
+// The slices of one launch: slice k is sino[k] -> img[k] (a contiguous batch
+// fills the pointers by stride).
+constexpr int kBpSlices = 64;
+struct BpSlices {
+    const float* sino[kBpSlices];
+    float* img[kBpSlices];
+};
+
 template <int TX, int TY, int PPT, int E, int U, int SB>
-__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(const float* __restrict__ sino,
-                                                                 float* __restrict__ img,
-                                                                 const float* __restrict__ cosT,
+__global__ void __launch_bounds__(TX* TY / PPT) bp_linear_kernel(BpSlices L, const float* __restrict__ cosT,
                                                                  const float* __restrict__ sinT, int slices,
                                                                  BpParams p)
 {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     int2 txy;
-    const long slice = (long)bp_work(p, txy) * SB;
+    const int first = bp_work(p, txy) * SB;
     const int x0 = txy.x * TX, y0 = txy.y * TY;
-    const long sino_stride = (long)p.P * p.D, img_stride = (long)p.N * p.N;
-    const float* s0 = sino + slice * sino_stride;
-    float* i0 = img + slice * img_stride;
     if constexpr (SB == 1) {
-        bp_tile<TX, TY, PPT, E, U, 1>(s0, i0, cosT, sinT, p, x0, y0, 0, smem);
+        bp_tile<TX, TY, PPT, E, U, 1>(L.sino[first], L.img[first], cosT, sinT, p, x0, y0, 0, smem);
     } else {
-        const int nv = (int)min((long)SB, (long)slices - slice);
-        bp_tile<TX, TY, PPT, E, U, SB>(s0, i0, cosT, sinT, p, x0, y0, 0, smem, sino_stride, img_stride, nv);
+        const int nv = min(SB, slices - first);
+        const float* sp[SB];
+        float* ip[SB];
+#pragma unroll
+        for (int b = 0; b < SB; b++) {
+            sp[b] = L.sino[b < nv ? first + b : first];
+            ip[b] = L.img[b < nv ? first + b : first];
+        }
+        bp_tile<TX, TY, PPT, E, U, SB>(sp[0], ip[0], cosT, sinT, p, x0, y0, 0, smem, sp, ip, nv);
     }
 }
 
@@ -1372,7 +1383,7 @@ namespace risa {
 // E * threads / W) angles so one thread stages at most E entries.
 struct BpVariant {
     int tx, ty, ppt, e, u, sb, amax;   // sb: slices per workgroup (1 or 2)
-    void (*fn)(const float*, float*, const float*, const float*, int, BpParams);
+    void (*fn)(BpSlices, const float*, const float*, int, BpParams);
 };
 
 #define RISA_BPV(TX, TY, PPT, E, U, SB, AMAX) {TX, TY, PPT, E, U, SB, AMAX, bp_linear_kernel<TX, TY, PPT, E, U, SB>}
@@ -1656,7 +1667,10 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
     });
 }
 
-static void launch_linear(risa_bp_t h, int variant, const float* sino, float* img, int slices, hipStream_t s)
+// Slice k of the call is sino(k) -> img(k); launches of up to kBpSlices
+// slices (a multiple of every variant's slices per workgroup).
+template <class SinoAt, class ImgAt>
+static void launch_linear_at(risa_bp_t h, int variant, SinoAt sino, ImgAt img, int slices, hipStream_t s)
 {
     const BpVariant& v = kVariants[variant];
     BpParams p = make_params(h, v.tx, v.ty);
@@ -1664,10 +1678,28 @@ static void launch_linear(risa_bp_t h, int variant, const float* sino, float* im
     p.AE = chunk_angles(threads, v.e, v.amax, p.W, v.sb);
     const size_t lds = tile_lds(p.AE, p.W, v.sb);
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
-    dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(slices, v.sb));
-    p.xg = xcd_groups(grid.x, grid.y);
-    hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
-    check_launch("bp_linear_kernel");
+    static_assert(kBpSlices % 4 == 0, "launch chunks hold whole slice groups");
+    for (int k0 = 0; k0 < slices; k0 += kBpSlices) {
+        const int n = std::min(kBpSlices, slices - k0);
+        BpSlices L{};
+        for (int k = 0; k < n; k++) {
+            L.sino[k] = sino(k0 + k);
+            L.img[k] = img(k0 + k);
+            if (!L.sino[k] || !L.img[k]) fail(RISA_ERR_ARG, "null slice pointer");
+        }
+        dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(n, v.sb));
+        p.xg = xcd_groups(grid.x, grid.y);
+        hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, L, h->cosT.ptr, h->sinT.ptr, n, p);
+        check_launch("bp_linear_kernel");
+    }
+}
+static void launch_linear(risa_bp_t h, int variant, const float* sino, float* img, int slices, hipStream_t s)
+{
+    const size_t ns = (size_t)h->g.numberOfParallelProjections * h->g.numberOfParallelDetectors;
+    const size_t ni = (size_t)h->g.numberOfPixels * h->g.numberOfPixels;
+    launch_linear_at(
+        h, variant, [&](int k) { return sino + (size_t)k * ns; }, [&](int k) { return img + (size_t)k * ni; }, slices,
+        s);
 }
 
 extern "C" int risa_hip_bp_run(risa_bp_t h, const float* sino, float* img, void* stream)
@@ -1694,6 +1726,20 @@ extern "C" int risa_hip_bp_run_batch(risa_bp_t h, const float* sino, int slices,
         if (h->g.useTextureMemory || h->g.interpolationType == 1)
             fail(RISA_ERR_UNSUPPORTED, "batched backprojection is linear interpolation only");
         if (slices) launch_linear(h, h->variant, sino, img, slices, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_bp_run_list(risa_bp_t h, const float* const* sino, int slices, float* const* img,
+                                    void* stream)
+{
+    return guarded("risa_hip_bp_run_list", [&] {
+        if (!h || !sino || !img || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (h->g.useTextureMemory || h->g.interpolationType == 1)
+            fail(RISA_ERR_UNSUPPORTED, "batched backprojection is linear interpolation only");
+        if (slices)
+            launch_linear_at(
+                h, h->variant, [&](int k) { return sino[k]; }, [&](int k) { return img[k]; }, slices,
+                as_stream(stream));
     });
 }
 

@@ -28,9 +28,10 @@ namespace risa {
 constexpr int kMaxFramesPerLaunch = 64;
 struct FrameGroups {
     int32_t plane[kMaxFramesPerLaunch];   // per group
-    int32_t start[kMaxFramesPerLaunch];   // first index into frame[]
+    int32_t start[kMaxFramesPerLaunch];   // first index into fan[] / par[]
     int32_t count[kMaxFramesPerLaunch];
-    int32_t frame[kMaxFramesPerLaunch];   // frame offsets within the launch
+    const float* fan[kMaxFramesPerLaunch];   // the launch's frames, grouped by plane
+    float* par[kMaxFramesPerLaunch];
 };
 
 // (Out of line: a slot gathers from global memory in a few per cent of the
@@ -87,9 +88,8 @@ __device__ __forceinline__ float mix_rays(bool r1, bool r2, float z1, float z2)
     return r1 ? z1 : (r2 ? z2 : 0.0f);
 }
 
-__global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fan, const uint4* __restrict__ table,
-                                                    const int4* __restrict__ boxes, const int2* __restrict__ order,
-                                                    float* __restrict__ par, int Df, int Pf, int D, int P,
+__global__ void __launch_bounds__(256) rebin_kernel(const uint4* __restrict__ table, const int4* __restrict__ boxes,
+                                                    const int2* __restrict__ order, int Df, int Pf, int D, int P,
                                                     long table_plane_stride, long box_plane_stride, FrameGroups fg)
 {
     __shared__ float patch[4 * kPatchMax];
@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
     }
     float pf[4][kPer];
     auto load_frame = [&](int u) {
-        const float* fr = fan + (long)fg.frame[first + u] * Df * Pf;
+        const float* __restrict__ fr = fg.fan[first + u];
 #pragma unroll
         for (int sl = 0; sl < 4; sl++)
 #pragma unroll
@@ -153,8 +153,7 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
     };
     if (n > 0) load_frame(0);
     for (int u = 0; u < n; u++) {
-        const int f = fg.frame[first + u];
-        const float* fr = fan + (long)f * Df * Pf;
+        const float* __restrict__ fr = fg.fan[first + u];
         if (u > 0) __syncthreads();   // the previous frame's patches are consumed
 #pragma unroll
         for (int sl = 0; sl < 4; sl++)
@@ -171,7 +170,7 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
                                    : 0.0f;
         const float zA = mix_rays((id[0] >> 31) != 0, (id[1] >> 31) != 0, z[0], z[1]);
         const float zB = mix_rays((id[2] >> 31) != 0, (id[3] >> 31) != 0, z[2], z[3]);
-        par[(long)f * D * P + e] = 0.5f * zA + 0.5f * zB;
+        fg.par[first + u][e] = 0.5f * zA + 0.5f * zB;
     }
 }
 
@@ -180,9 +179,8 @@ __global__ void __launch_bounds__(256) rebin_kernel(const float* __restrict__ fa
 // second-half samples there, cuda_kernels_fan2para.h:164-176).  One thread per
 // row j adds it after rebin_kernel has written the row, in the reference's
 // order (first-half sample, then i = D/2, then i = D - 1).
-__global__ void __launch_bounds__(256) rebin_extra_kernel(const float* __restrict__ fan,
-                                                          const uint4* __restrict__ extra, float* __restrict__ par,
-                                                          int Df, int Pf, int D, int P, FrameGroups fg)
+__global__ void __launch_bounds__(256) rebin_extra_kernel(const uint4* __restrict__ extra, int Df, int Pf, int D,
+                                                          int P, FrameGroups fg)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int grp = blockIdx.y;
@@ -194,11 +192,10 @@ __global__ void __launch_bounds__(256) rebin_extra_kernel(const float* __restric
     if (!r1 && !r2) return;   // adds +0.0f: the output keeps its bits
     const long e = (long)j * D + (D / 2 - 1);
     for (int u = 0; u < fg.count[grp]; u++) {
-        const int f = fg.frame[fg.start[grp] + u];
-        const float* fr = fan + (long)f * Df * Pf;
+        const float* __restrict__ fr = fg.fan[fg.start[grp] + u];
         const float z1 = r1 ? rebin_ray(fr, idx.x, __uint_as_float(ftb.x), __uint_as_float(fgb.x), Df, Pf) : 0.0f;
         const float z2 = r2 ? rebin_ray(fr, idx.y, __uint_as_float(ftb.y), __uint_as_float(fgb.y), Df, Pf) : 0.0f;
-        float* out = par + (long)f * D * P + e;
+        float* out = fg.par[fg.start[grp] + u] + e;
         *out = *out + 0.5f * mix_rays(r1, r2, z1, z2);
     }
 }
@@ -441,8 +438,9 @@ extern "C" int risa_hip_fan2para_create(const risa_geom* g, int device, risa_fan
     return risa_hip_fan2para_create_ex(g, device, RISA_REBIN_WEIGHTS_FP64, out);
 }
 
-static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* planes, int frames, float* par,
-                         hipStream_t s)
+// Frame f of the call reads fan(f) and writes par(f).
+template <class FanAt, class ParAt>
+static void launch_rebin_at(risa_fan2para_t h, FanAt fan, const int32_t* planes, int frames, ParAt par, hipStream_t s)
 {
     const int Df = h->g.numberOfFanDetectors, Pf = h->g.numberOfFanProjections;
     const int D = h->g.numberOfParallelDetectors, P = h->g.numberOfParallelProjections;
@@ -455,7 +453,12 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
             for (int f = 0; f < nf; f++) {
                 const int pl = planes[f0 + f];
                 if (pl < 0 || pl >= h->g.numberOfPlanes) fail(RISA_ERR_ARG, "plane out of range");
-                if (pl == k) fg.frame[pos + cnt++] = f;
+                if (pl == k) {
+                    fg.fan[pos + cnt] = fan(f0 + f);
+                    fg.par[pos + cnt] = par(f0 + f);
+                    if (!fg.fan[pos + cnt] || !fg.par[pos + cnt]) fail(RISA_ERR_ARG, "null frame pointer");
+                    cnt++;
+                }
             }
             if (cnt) {
                 fg.plane[groups] = k;
@@ -466,17 +469,24 @@ static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* pla
             }
         }
         dim3 grid(ceil_div(D, kRebinBlock) * ceil_div(P, kRebinBlock), groups);
-        hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, s, fan + (size_t)f0 * Df * Pf,
-                           reinterpret_cast<const uint4*>(h->table.ptr), h->boxes.ptr, h->order.ptr,
-                           par + (size_t)f0 * D * P, Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
+        hipLaunchKernelGGL(rebin_kernel, grid, dim3(256), 0, s, reinterpret_cast<const uint4*>(h->table.ptr),
+                           h->boxes.ptr, h->order.ptr, Df, Pf, D, P, (long)(h->plane_words / 4), h->box_plane, fg);
         check_launch("rebin_kernel");
         if (D % 2 != 0) {
             hipLaunchKernelGGL(rebin_extra_kernel, dim3(ceil_div(P, 256), groups), dim3(256), 0, s,
-                               fan + (size_t)f0 * Df * Pf, reinterpret_cast<const uint4*>(h->extra.ptr),
-                               par + (size_t)f0 * D * P, Df, Pf, D, P, fg);
+                               reinterpret_cast<const uint4*>(h->extra.ptr), Df, Pf, D, P, fg);
             check_launch("rebin_extra_kernel");
         }
     }
+}
+static void launch_rebin(risa_fan2para_t h, const float* fan, const int32_t* planes, int frames, float* par,
+                         hipStream_t s)
+{
+    const size_t nf = (size_t)h->g.numberOfFanDetectors * h->g.numberOfFanProjections;
+    const size_t np = (size_t)h->g.numberOfParallelDetectors * h->g.numberOfParallelProjections;
+    launch_rebin_at(
+        h, [&](int f) { return fan + (size_t)f * nf; }, planes, frames, [&](int f) { return par + (size_t)f * np; },
+        s);
 }
 
 extern "C" int risa_hip_fan2para_run(risa_fan2para_t h, const float* fan, int plane, float* par, void* stream)
@@ -494,6 +504,17 @@ extern "C" int risa_hip_fan2para_run_batch(risa_fan2para_t h, const float* fan, 
     return guarded("risa_hip_fan2para_run_batch", [&] {
         if (!h || !fan || !par || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
         if (frames) launch_rebin(h, fan, planes, frames, par, as_stream(stream));
+    });
+}
+
+extern "C" int risa_hip_fan2para_run_list(risa_fan2para_t h, const float* const* fan, const int32_t* planes,
+                                          int frames, float* const* par, void* stream)
+{
+    return guarded("risa_hip_fan2para_run_list", [&] {
+        if (!h || !fan || !par || !planes || frames < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (frames)
+            launch_rebin_at(
+                h, [&](int f) { return fan[f]; }, planes, frames, [&](int f) { return par[f]; }, as_stream(stream));
     });
 }
 

@@ -125,14 +125,24 @@ __device__ __forceinline__ void stockham_pass(const float2* src, float2* dst, in
     }
 }
 
+// The rows of one FFT-filter launch: row r is row r % per of frame r / per,
+// read from in[r / per] and written to out[r / per] (a contiguous call is one
+// frame of all its rows; out null: FP16 output only).
+constexpr int kFilterFrames = 64;
+struct RowFrames {
+    const float* in[kFilterFrames];
+    float* out[kFilterFrames];
+    int per;
+    int has_out;
+};
+
 // rows [rows][D] in -> out (FP32 and/or FP16*out_scale); rows are paired.
 // The transform length M = plan.D is D itself, or (for lengths the radices
 // do not factor) an M >= 2D - 1 over which the length-D circular convolution
 // is exact: the row is zero-padded to M, H is the response of the wrapped
 // circulant on M points, and the first D outputs are kept.  A workgroup of
 // blockDim.x / 128 slots holds one row pair per slot.
-__global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                         uint16_t* __restrict__ out16, float out_scale,
+__global__ void __launch_bounds__(256) fft_filter_kernel(RowFrames fr, uint16_t* __restrict__ out16, float out_scale,
                                                          const float2* __restrict__ tw, const float* __restrict__ H,
                                                          FftPlan plan, int D, long rows)
 {
@@ -148,9 +158,11 @@ __global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict
     float2* b1 = b0 + M;
     const bool active = ra < rows;
     const bool hasb = rb < rows;
+    const int fa = (int)(ra / fr.per), fb = hasb ? (int)(rb / fr.per) : fa;
+    const long qa = ra - (long)fa * fr.per, qb = rb - (long)fb * fr.per;
     if (active) {
-        const float* xa = in + ra * D;
-        const float* xb = in + rb * D;
+        const float* xa = fr.in[fa] + qa * D;
+        const float* xb = fr.in[fb] + qb * D;
         for (int n = t0; n < D; n += T) b0[n] = make_float2(xa[n], hasb ? xb[n] : 0.0f);
         for (int n = D + t0; n < M; n += T) b0[n] = make_float2(0.0f, 0.0f);
     }
@@ -177,9 +189,9 @@ __global__ void __launch_bounds__(256) fft_filter_kernel(const float* __restrict
     if (!active) return;
     for (int n = t0; n < D; n += T) {
         const float2 v = src[n];
-        if (out) {
-            out[ra * D + n] = v.x;
-            if (hasb) out[rb * D + n] = v.y;
+        if (fr.has_out) {
+            fr.out[fa][qa * D + n] = v.x;
+            if (hasb) fr.out[fb][qb * D + n] = v.y;
         }
         if (out16) {
             out16[ra * D + n] = float_to_half_bits(v.x * out_scale);
@@ -228,8 +240,7 @@ __device__ __forceinline__ void fft512_pass(float2 (&v)[8], float2* buf, const f
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(64) fft512_filter_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                           uint16_t* __restrict__ out16, float out_scale,
+__global__ void __launch_bounds__(64) fft512_filter_kernel(RowFrames fr, uint16_t* __restrict__ out16, float out_scale,
                                                            const float2* __restrict__ tw,
                                                            const float* __restrict__ H, long rows)
 {
@@ -237,8 +248,10 @@ __global__ void __launch_bounds__(64) fft512_filter_kernel(const float* __restri
     const int L = threadIdx.x;
     const long ra = (long)blockIdx.x * 2, rb = ra + 1;
     const bool hasb = rb < rows;
-    const float* xa = in + ra * 512;
-    const float* xb = in + rb * 512;
+    const int fa = (int)(ra / fr.per), fb = hasb ? (int)(rb / fr.per) : fa;
+    const long qa = ra - (long)fa * fr.per, qb = rb - (long)fb * fr.per;
+    const float* xa = fr.in[fa] + qa * 512;
+    const float* xb = fr.in[fb] + qb * 512;
     float2 v[8], w1[8], w2[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) v[c] = make_float2(xa[L + 64 * c], hasb ? xb[L + 64 * c] : 0.0f);
@@ -258,9 +271,9 @@ __global__ void __launch_bounds__(64) fft512_filter_kernel(const float* __restri
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const long n = L + 64 * c;
-        if (out) {
-            out[ra * 512 + n] = v[c].x;
-            if (hasb) out[rb * 512 + n] = v[c].y;
+        if (fr.has_out) {
+            fr.out[fa][qa * 512 + n] = v[c].x;
+            if (hasb) fr.out[fb][qb * 512 + n] = v[c].y;
         }
         if (out16) {
             out16[ra * 512 + n] = float_to_half_bits(v[c].x * out_scale);
@@ -1056,13 +1069,13 @@ extern "C" int risa_hip_filter_get_taps(risa_filter_t h, float* taps)
     });
 }
 
-static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* out16, float out_scale, long rows,
+static void launch_fft(risa_filter_t h, const RowFrames& fr, uint16_t* out16, float out_scale, long rows,
                        hipStream_t s)
 {
     const int D = h->g.numberOfParallelDetectors, M = h->M;
     const long pairs = (rows + 1) / 2;
     if (D == 512 && M == 512) {
-        hipLaunchKernelGGL(fft512_filter_kernel, dim3((unsigned)pairs), dim3(64), 0, s, in, out, out16, out_scale,
+        hipLaunchKernelGGL(fft512_filter_kernel, dim3((unsigned)pairs), dim3(64), 0, s, fr, out16, out_scale,
                            h->tw.ptr, h->H.ptr, rows);
         check_launch("fft512_filter_kernel");
         return;
@@ -1070,9 +1083,21 @@ static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* o
     const int slots = fft_slots(M);
     const int grid = ceil_div(pairs, slots);
     const size_t lds = (size_t)slots * 2 * M * sizeof(float2);
-    hipLaunchKernelGGL(fft_filter_kernel, dim3(grid), dim3(128 * slots), lds, s, in, out, out16, out_scale, h->tw.ptr,
+    hipLaunchKernelGGL(fft_filter_kernel, dim3(grid), dim3(128 * slots), lds, s, fr, out16, out_scale, h->tw.ptr,
                        h->H.ptr, h->plan, D, rows);
     check_launch("fft_filter_kernel");
+}
+// `rows` consecutive rows in -> out (out null: FP16 output only).
+static void launch_fft(risa_filter_t h, const float* in, float* out, uint16_t* out16, float out_scale, long rows,
+                       hipStream_t s)
+{
+    if (rows > 0x7fffffffL) fail(RISA_ERR_UNSUPPORTED, "too many rows");
+    RowFrames fr{};
+    fr.in[0] = in;
+    fr.out[0] = out;
+    fr.per = (int)std::max(1L, rows);
+    fr.has_out = out != nullptr;
+    launch_fft(h, fr, out16, out_scale, rows, s);
 }
 
 static void launch_gemm(risa_filter_t h, const float* in, GemmOut o, long rows, hipStream_t s)
@@ -1126,6 +1151,27 @@ extern "C" int risa_hip_filter_run(risa_filter_t h, float* sino, int slices, voi
                 RISA_HIP_CHECK(hipMemcpyAsync(x, h->stage.ptr, (size_t)n * D * sizeof(float), hipMemcpyDeviceToDevice,
                                               as_stream(stream)));
             }
+        }
+    });
+}
+
+extern "C" int risa_hip_filter_run_list(risa_filter_t h, float* const* sino, int slices, void* stream)
+{
+    return guarded("risa_hip_filter_run_list", [&] {
+        if (!h || !sino || slices < 0) fail(RISA_ERR_ARG, "bad argument");
+        if (h->mode != RISA_FILTER_MODE_FFT32) fail(RISA_ERR_UNSUPPORTED, "frame lists are filtered in FFT32 mode");
+        const int P = h->g.numberOfParallelProjections;
+        for (int f0 = 0; f0 < slices; f0 += kFilterFrames) {
+            const int nf = std::min(kFilterFrames, slices - f0);
+            RowFrames fr{};
+            for (int f = 0; f < nf; f++) {
+                if (!sino[f0 + f]) fail(RISA_ERR_ARG, "null frame pointer");
+                fr.in[f] = sino[f0 + f];
+                fr.out[f] = sino[f0 + f];
+            }
+            fr.per = P;
+            fr.has_out = 1;
+            launch_fft(h, fr, nullptr, 1.0f, (long)nf * P, as_stream(stream));
         }
     });
 }

@@ -55,6 +55,12 @@ int processors_per_device(const std::string& configFile)
     return n < 1 ? 1 : n;
 }
 
+int frame_batch(const std::string& configFile)
+{
+    const int n = config_int_or(configFile, "frameBatch", 1);
+    return n < 1 ? 1 : n;
+}
+
 risa_geom read_geometry(const std::string& configFile)
 {
     risa_geom g{};
@@ -241,13 +247,22 @@ void Attenuation::init(const std::string& configFile, const std::vector<float>& 
     start_processors(configFile, pool, devices, lanes);
 }
 
-void Attenuation::start_processors(const std::string&, int pool, int devices, int lanes)
+void Attenuation::start_processors(const std::string& configFile, int pool, int devices, int lanes)
 {
-    start("Attenuation", devices, lanes, pool, fan_elems(g_),
-          [this](int w, int, input_type& in, output_type& out, hipStream_t s) {
-              check(risa_hip_attenuation_run(h_[(std::size_t)w], in.data(), (int)in.plane(), out.data(), s),
-                    "Attenuation::processor");
-          });
+    start_batched("Attenuation", devices, lanes, pool, fan_elems(g_), frame_batch(configFile),
+                  [this](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      std::vector<const uint16_t*> raw((std::size_t)n);
+                      std::vector<float*> out((std::size_t)n);
+                      std::vector<int32_t> planes((std::size_t)n);
+                      for (int i = 0; i < n; i++) {
+                          raw[(std::size_t)i] = ins[i].data();
+                          out[(std::size_t)i] = outs[i].data();
+                          planes[(std::size_t)i] = (int32_t)ins[i].plane();
+                      }
+                      check(risa_hip_attenuation_run_list(h_[(std::size_t)w], raw.data(), planes.data(), n,
+                                                          out.data(), s),
+                            "Attenuation::processor");
+                  });
 }
 
 Attenuation::~Attenuation()
@@ -268,11 +283,20 @@ Fan2Para::Fan2Para(const std::string& configFile)
         check(risa_hip_fan2para_create(&g_, d, &h), "Fan2Para");
         h_.push_back(h);
     }
-    start("Fan2Para", devices, lanes, pool, par_elems(g_),
-          [this, lanes](int w, int, input_type& in, output_type& out, hipStream_t s) {
-              check(risa_hip_fan2para_run(h_[(std::size_t)(w / lanes)], in.data(), (int)in.plane(), out.data(), s),
-                    "Fan2Para::processor");
-          });
+    start_batched("Fan2Para", devices, lanes, pool, par_elems(g_), frame_batch(configFile),
+                  [this, lanes](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      std::vector<const float*> fan((std::size_t)n);
+                      std::vector<float*> par((std::size_t)n);
+                      std::vector<int32_t> planes((std::size_t)n);
+                      for (int i = 0; i < n; i++) {
+                          fan[(std::size_t)i] = ins[i].data();
+                          par[(std::size_t)i] = outs[i].data();
+                          planes[(std::size_t)i] = (int32_t)ins[i].plane();
+                      }
+                      check(risa_hip_fan2para_run_list(h_[(std::size_t)(w / lanes)], fan.data(), planes.data(), n,
+                                                       par.data(), s),
+                            "Fan2Para::processor");
+                  });
 }
 
 Fan2Para::~Fan2Para()
@@ -291,9 +315,12 @@ Filter::Filter(const std::string& configFile)
         check(risa_hip_filter_create(&g_, w / lanes, RISA_FILTER_MODE_FFT32, &h), "Filter");
         h_.push_back(h);
     }
-    start("Filter", devices, lanes, 0, 0, [this](int w, int, input_type& img, output_type&, hipStream_t s) {
-        check(risa_hip_filter_run(h_[(std::size_t)w], img.data(), 1, s), "Filter::processor");
-    });
+    start_batched("Filter", devices, lanes, 0, 0, frame_batch(configFile),
+                  [this](int w, int, input_type* ins, output_type*, int n, hipStream_t s) {
+                      std::vector<float*> sino((std::size_t)n);
+                      for (int i = 0; i < n; i++) sino[(std::size_t)i] = ins[i].data();
+                      check(risa_hip_filter_run_list(h_[(std::size_t)w], sino.data(), n, s), "Filter::processor");
+                  });
 }
 
 Filter::~Filter()
@@ -413,7 +440,6 @@ Backprojection::Backprojection(const std::string& configFile)
     // reconstruct two slices per workgroup; the tile shape is timed at the
     // frames the device's lanes then keep in flight
     batch_ = std::max(1, std::min(config_int_or(configFile, "backProjectionBatch", 4), std::max(1, pool)));
-    const std::size_t plane = (std::size_t)g_.numberOfParallelProjections * g_.numberOfParallelDetectors;
     for (int w = 0; w < devices * lanes; w++) {
         risa_bp_t h = nullptr;
         check(risa_hip_bp_create(&g_, w / lanes, &h), "Backprojection");
@@ -430,35 +456,17 @@ Backprojection::Backprojection(const std::string& configFile)
         }
         check(risa_hip_bp_get_variant(h_[(std::size_t)(w - w % lanes)], &variant), "Backprojection variant");
         check(risa_hip_bp_set_variant(h, variant), "Backprojection variant");
-        // per processor: contiguous staging for a batch of sinograms and images
-        float* in32 = nullptr;
-        float* out32 = nullptr;
-        if (batch_ > 1) {
-            hip_check(hipMalloc(&in32, (std::size_t)batch_ * plane * sizeof(float)), "Backprojection staging");
-            hip_check(hipMalloc(&out32, (std::size_t)batch_ * img_elems(g_) * sizeof(float)), "Backprojection staging");
-        }
-        in32_.push_back(in32);
-        out32_.push_back(out32);
     }
     start_batched("Backprojection", devices, lanes, pool, img_elems(g_), batch_,
-                  [this, plane](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
-                      const risa_bp_t h = h_[(std::size_t)w];
-                      if (n == 1) {
-                          check(risa_hip_bp_run(h, ins[0].data(), outs[0].data(), s), "Backprojection::processor");
-                          return;
+                  [this](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      std::vector<const float*> sino((std::size_t)n);
+                      std::vector<float*> img((std::size_t)n);
+                      for (int i = 0; i < n; i++) {
+                          sino[(std::size_t)i] = ins[i].data();
+                          img[(std::size_t)i] = outs[i].data();
                       }
-                      float* in32 = in32_[(std::size_t)w];
-                      float* out32 = out32_[(std::size_t)w];
-                      const std::size_t img = img_elems(g_);
-                      for (int i = 0; i < n; i++)
-                          hip_check(hipMemcpyAsync(in32 + (std::size_t)i * plane, ins[i].data(), plane * sizeof(float),
-                                                   hipMemcpyDeviceToDevice, s),
-                                    "Backprojection gather");
-                      check(risa_hip_bp_run_batch(h, in32, n, out32, s), "Backprojection::processor");
-                      for (int i = 0; i < n; i++)
-                          hip_check(hipMemcpyAsync(outs[i].data(), out32 + (std::size_t)i * img, img * sizeof(float),
-                                                   hipMemcpyDeviceToDevice, s),
-                                    "Backprojection scatter");
+                      check(risa_hip_bp_run_list(h_[(std::size_t)w], sino.data(), n, img.data(), s),
+                            "Backprojection::processor");
                   });
 }
 
@@ -466,10 +474,6 @@ Backprojection::~Backprojection()
 {
     stop();
     for (auto h : h_) risa_hip_bp_destroy(h);
-    for (auto p : in32_)
-        if (p) (void)hipFree(p);
-    for (auto p : out32_)
-        if (p) (void)hipFree(p);
 }
 
 // -------------------------------------------------------------- Masking --

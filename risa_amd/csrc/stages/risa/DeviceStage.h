@@ -55,6 +55,12 @@ int device_count();
 // processorsPerDevice, default 1 (the reference runs one per device).
 int processors_per_device(const std::string& configFile);
 
+// Frames a processor of the Attenuation, Fan2Para and Filter stages takes
+// from its queue at once (those already queued, none waited for) and runs
+// in one launch: the optional key frameBatch, default 1 (the reference's
+// one frame per call).
+int frame_batch(const std::string& configFile);
+
 template <class InT, class OutT>
 class DeviceStage {
 public:
@@ -132,14 +138,15 @@ protected:
         for (auto& w : workers_) w->thread = std::thread(&DeviceStage::loop, this, w.get());
     }
 
-    // Like start() for an allocating stage, but a processor takes a frame
-    // and then, without waiting, up to max_batch - 1 more that are already
-    // queued, and hands them to fn together (kernels that share work across
-    // slices).  Frames still leave in the order they were taken.
+    // Like start(), but a processor takes a frame and then, without waiting,
+    // up to max_batch - 1 more that are already queued, and hands them to fn
+    // together (one launch for the batch).  Frames still leave in the order
+    // they were taken.  In-place stages (out_elements 0) get the inputs as
+    // the outputs.
     void start_batched(const char* name, int devices, int lanes, int pool_size, std::size_t out_elements,
                        int max_batch, batch_fn fn)
     {
-        max_batch_ = std::max(1, std::min(max_batch, pool_size));
+        max_batch_ = std::max(1, out_elements > 0 ? std::min(max_batch, pool_size) : max_batch);
         bfn_ = std::move(fn);
         start(name, devices, lanes, pool_size, out_elements, nullptr);
     }
@@ -247,6 +254,14 @@ private:
                     break;
                 }
                 ins.push_back(std::move(more));
+            }
+            if constexpr (std::is_same<input_type, output_type>::value) {
+                if (!allocating_) {
+                    bfn_(w->index, w->device, ins.data(), ins.data(), (int)ins.size(), w->stream);
+                    hip_check(hipStreamSynchronize(w->stream), name_);
+                    for (auto& in : ins) results_.push(std::move(in));
+                    continue;
+                }
             }
             for (std::size_t i = 0; i < ins.size(); i++) outs.push_back(pool->requestMemory(w->pool));
             bfn_(w->index, w->device, ins.data(), outs.data(), (int)ins.size(), w->stream);

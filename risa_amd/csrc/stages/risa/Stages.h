@@ -147,7 +147,7 @@ private:
 bool mixed_precision(const std::string& configFile);
 
 // FP32 backprojection with the fused mask; up to backProjectionBatch (default
-// 4) queued frames per launch.
+// 4) queued frames per launch (risa_hip_bp_run_list: no copies).
 class Backprojection : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {
 public:
     explicit Backprojection(const std::string& configFile);
@@ -157,8 +157,6 @@ private:
     risa_geom g_{};
     int batch_ = 4;                // frames per launch (key backProjectionBatch)
     std::vector<risa_bp_t> h_;
-    std::vector<float*> in32_;     // per processor: [batch][P][D] staging
-    std::vector<float*> out32_;    // per processor: [batch][N][N] staging
 };
 
 class Masking : public DeviceStage<DeviceImage<float>, DeviceImage<float>> {

@@ -50,6 +50,12 @@ class _Handle:
         self.close()
 
 
+def _ptr_list(tensors):
+    """A host array of the device addresses of `tensors` (for the *_run_list entries)."""
+    arr = (C.c_void_p * max(1, len(tensors)))(*[t.data_ptr() for t in tensors])
+    return arr
+
+
 def _require_cuda(*tensors):
     for t in tensors:
         if t is not None and not t.is_cuda:
@@ -82,6 +88,14 @@ class Attenuation(_Handle):
         pl = np.ascontiguousarray(planes, dtype=np.int32)
         check(load().risa_hip_attenuation_run_batch(self.h, ptr(raw), ptr(pl), len(pl), ptr(out), _stream()))
         return out
+
+    def run_list(self, raws, planes, outs):
+        """Separately allocated frames raws[f] -> outs[f]."""
+        _require_cuda(*raws, *outs)
+        pl = np.ascontiguousarray(planes, dtype=np.int32)
+        a, b = _ptr_list(raws), _ptr_list(outs)
+        check(load().risa_hip_attenuation_run_list(self.h, C.addressof(a), ptr(pl), len(pl), C.addressof(b), _stream()))
+        return outs
 
     @property
     def input_layout(self) -> int:
@@ -138,6 +152,14 @@ class Fan2Para(_Handle):
         check(load().risa_hip_fan2para_run_batch(self.h, ptr(fan), ptr(pl), len(pl), ptr(out), _stream()))
         return out
 
+    def run_list(self, fans, planes, outs):
+        """Separately allocated frames fans[f] -> outs[f]."""
+        _require_cuda(*fans, *outs)
+        pl = np.ascontiguousarray(planes, dtype=np.int32)
+        a, b = _ptr_list(fans), _ptr_list(outs)
+        check(load().risa_hip_fan2para_run_list(self.h, C.addressof(a), ptr(pl), len(pl), C.addressof(b), _stream()))
+        return outs
+
     def table_bytes(self) -> int:
         n = C.c_size_t()
         check(load().risa_hip_fan2para_table_bytes(self.h, C.byref(n)))
@@ -157,6 +179,13 @@ class Filter(_Handle):
         _require_cuda(sino)
         check(load().risa_hip_filter_run(self.h, ptr(sino), slices, _stream()))
         return sino
+
+    def run_list(self, sinos):
+        """In place over separately allocated sinograms (FFT32 mode)."""
+        _require_cuda(*sinos)
+        a = _ptr_list(sinos)
+        check(load().risa_hip_filter_run_list(self.h, C.addressof(a), len(sinos), _stream()))
+        return sinos
 
     def f16(self, sino, out16, slices: int, out_scale: float):
         _require_cuda(sino, out16)
@@ -229,6 +258,13 @@ class Backprojection(_Handle):
         _require_cuda(sino, img)
         check(load().risa_hip_bp_run_batch(self.h, ptr(sino), slices, ptr(img), _stream()))
         return img
+
+    def run_list(self, sinos, imgs):
+        """Separately allocated slices sinos[k] -> imgs[k]."""
+        _require_cuda(*sinos, *imgs)
+        a, b = _ptr_list(sinos), _ptr_list(imgs)
+        check(load().risa_hip_bp_run_list(self.h, C.addressof(a), len(sinos), C.addressof(b), _stream()))
+        return imgs
 
     def f16(self, sino16, in_scale: float, slices: int, img):
         _require_cuda(sino16, img)
