@@ -9,11 +9,13 @@
 // keeps the first few slices for checking.
 //
 //   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] [--tiff]
-//                 [--skip-every K]
+//                 [--skip-every K] [--sync spin|yield|block]
 //
 // precision = "mixed" in the configuration wires FilterMixed and
 // BackprojectionMixed (FP16 filtered sinograms) in place of Filter and
 // Backprojection.
+// --sync: how the stages' host threads wait for their streams (the device's
+// scheduling flag, set before any other HIP call; default: the runtime's).
 // --skip-every K: the source leaves out every index i with i % K == K - 1
 // (frames lost before the GPU), which H2D's accounting counts.
 //
@@ -167,6 +169,7 @@ struct Options {
     int unique = 8, keep = 0;
     long skip_every = 0;
     bool masking = false, module_major = false, tiff = false;
+    unsigned sync_flag = 0;   // hipDeviceSchedule*, 0: the runtime's default
     std::string save_path;
 };
 
@@ -238,7 +241,7 @@ int main(int argc, char** argv)
     if (argc < 2) {
         std::fprintf(stderr,
                      "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] "
-                     "[--tiff] [--skip-every K]\n",
+                     "[--tiff] [--skip-every K] [--sync spin|yield|block]\n",
                      argv[0]);
         return 2;
     }
@@ -255,7 +258,16 @@ int main(int argc, char** argv)
         else if (a == "--module-major") o.module_major = true;
         else if (a == "--tiff") o.tiff = true;
         else if (a == "--skip-every" && i + 1 < argc) o.skip_every = std::atol(argv[++i]);
-        else {
+        else if (a == "--sync" && i + 1 < argc) {
+            const std::string m = argv[++i];
+            if (m == "spin") o.sync_flag = hipDeviceScheduleSpin;
+            else if (m == "yield") o.sync_flag = hipDeviceScheduleYield;
+            else if (m == "block") o.sync_flag = hipDeviceScheduleBlockingSync;
+            else {
+                std::fprintf(stderr, "--sync takes spin, yield or block\n");
+                return 2;
+            }
+        } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
@@ -265,6 +277,16 @@ int main(int argc, char** argv)
         return 2;
     }
 
+    if (o.sync_flag) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        for (int d = 0; d < n; d++)
+            if (hipSetDevice(d) != hipSuccess || hipSetDeviceFlags(o.sync_flag) != hipSuccess) {
+                std::fprintf(stderr, "cannot set the scheduling flag on device %d\n", d);
+                return 1;
+            }
+        if (n > 0) (void)hipSetDevice(0);
+    }
     Provenance prov;
     SinkState st;
     st.keep = o.keep;
