@@ -453,7 +453,7 @@ def main():
                        "forward_us_per_slice": round(times["forward"] * 1e3 / S4, 1),
                        "backward_us_per_slice": round(times["backward"] * 1e3 / S4, 1),
                        "what": "C4 geometry; forward = filter + batched backprojection; backward = "
-                               "bp_adjoint_path_kernel (batched, 8 slices per pass) + fft_grad_taps_kernel + "
+                               "bp_adjoint_path_kernel (batched, 16 slices per pass, 2 per wave) + fft_grad_taps_kernel + "
                                "conjugate filter; one step of the autograd function"}
             log(f"differentiable FBP: {adjoint}")
         except Exception as exc:  # reported, not fatal

@@ -331,7 +331,7 @@ int risa_hip_bp_run_f16x4(risa_bp_t h, const uint16_t* sino_x4, float in_scale, 
  * path's backward.) */
 int risa_hip_bp_adjoint_run(risa_bp_t h, const float* img, float* sino, void* stream);
 /* The same for `slices` images [slices][N][N] -> [slices][P][D] in passes of
- * up to 8 slices per launch sequence through buffers sized at create. */
+ * up to 16 slices per launch sequence through buffers sized at create. */
 int risa_hip_bp_adjoint_run_batch(risa_bp_t h, const float* img, int slices, float* sino, void* stream);
 /* Adjoint method: 0 (default) the sheared-path kernel where it applies,
  * 1 the ray-driven gather everywhere (same operator, slower; for comparison

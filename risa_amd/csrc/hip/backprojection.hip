@@ -1154,16 +1154,20 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One lane's path over rows [r0, r1] of angle p (class CLS); returns its
-// three sums.  tab: this wave's segment table.  rt[r] = {byte offset of the
-// path's row-r pixel of lane 0 in the padded image, minus J0 * 4; delta(r) * 8}.
-// Rows go in blocks of 16 (8 where a segment has fewer rows left; a partial
-// block's spare rows add zero), loads first.
-template <int CLS>
-__device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, const int2* __restrict__ rt,
+// One lane's path over rows [r0, r1] of angle p (class CLS) through AS
+// images at once (rsrc[i]); returns each image's three sums.  The path, its
+// table entries, the pixel coordinate and the fraction are shared; each
+// image adds one buffer load and three accumulations per row, with the same
+// operations as a one-image path.  tab: this wave's segment table.  rt[r] =
+// {byte offset of the path's row-r pixel of lane 0 in the padded image,
+// minus J0 * 4; delta(r) * 8}.  Rows go in blocks of 16 / AS (8 / AS where a
+// segment has fewer rows left; a partial block's spare rows add zero),
+// loads first.
+template <int CLS, int AS>
+__device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t (&rsrc)[AS], const int2* __restrict__ rt,
                                          const float* __restrict__ coord, const float* __restrict__ Tp,
                                          const AdjParams& q, float2* tab, int J0, int lane, int r0, int r1, float c,
-                                         float s, float A1, float& g0, float& gm, float& ga)
+                                         float s, float A1, float (&g0)[AS], float (&gm)[AS], float (&ga)[AS])
 {
     // the segment: rows of one forward tile row (class A) or column (class B)
     const int sh = CLS ? q.lgTX : q.lgTY;
@@ -1178,10 +1182,10 @@ __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, cons
         const lds_f2 w = lds_load<lds_f2>(a);
         return make_float2(w.x, w.y);
     };
-    auto load = [&](int off) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, off + J04, 0));
+    auto load = [&](int k, int off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc[k], voff, off + J04, 0));
     };
-    auto row = [&](float v, float2 e, float rc) {
+    auto row = [&](const float (&v)[AS], float2 e, float rc) {
         float tp, rm;
         if (CLS == 0) {
             tp = __builtin_fmaf(rc, s, e.x);                           // fma(yp, sin, xc)
@@ -1191,22 +1195,28 @@ __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, cons
             rm = -e.y - A1;
         }
         const float m = tp + rm;
-        g0 += v;
-        gm = __builtin_fmaf(v, m, gm);
-        ga = __builtin_fmaf(v, fabsf(m), ga);   // |m|: a source modifier, no extra operation
+#pragma unroll
+        for (int k = 0; k < AS; k++) {
+            g0[k] += v[k];
+            gm[k] = __builtin_fmaf(v[k], m, gm[k]);
+            ga[k] = __builtin_fmaf(v[k], fabsf(m), ga[k]);   // |m|: a source modifier, no extra operation
+        }
     };
     // NB rows from r, of which the first `live` count (the rest re-read row
     // r + live - 1, in bounds, and add zero)
     auto block = [&](auto nb, int r, int live) {
         constexpr int NB = decltype(nb)::value;
-        float v[NB], rc[NB];
+        float v[NB][AS], rc[NB];
         float2 e[NB];
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const int ri = r + min(i, live - 1);
             const int2 t = rt[ri];
-            v[i] = load(t.x);
-            if (i >= live) v[i] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < AS; k++) {
+                v[i][k] = load(k, t.x);
+                if (i >= live) v[i][k] = 0.0f;
+            }
             e[i] = entry(t.y);
             rc[i] = coord[ri];
         }
@@ -1240,28 +1250,32 @@ __device__ __forceinline__ void adj_path(const __amdgpu_buffer_rsrc_t rsrc, cons
         }
         // rows left in this segment (segments are whole tile rows / columns)
         const int lim = min(r1, segend) - r + 1;
-        if (lim >= 16) {
-            block(std::integral_constant<int, 16>{}, r, 16);
-            r += 16;
+        constexpr int BIG = 16 / AS > 4 ? 16 / AS : 4, SMALL = BIG / 2;
+        if (lim >= BIG) {
+            block(std::integral_constant<int, BIG>{}, r, BIG);
+            r += BIG;
         } else {
-            block(std::integral_constant<int, 8>{}, r, min(lim, 8));
-            r += min(lim, 8);
+            block(std::integral_constant<int, SMALL>{}, r, min(lim, SMALL));
+            r += min(lim, SMALL);
         }
     }
 }
 
+// Workgroup (x, y, z): kAdjWavesPerGroup angles of the sorted list, band y,
+// images AS z .. AS z + AS - 1 of the pass (those below `slices`).
+template <int AS>
 __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel(
     const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ partial,
     const AdjAngle* __restrict__ angles, const int2* __restrict__ bands, const int2* __restrict__ rowtab,
     const float* __restrict__ coord, const float* __restrict__ Ttab, const float* __restrict__ cosT,
-    const float* __restrict__ sinT, AdjParams q)
+    const float* __restrict__ sinT, int slices, AdjParams q)
 {
     __shared__ float win[kAdjWavesPerGroup][kAdjWin];
     __shared__ float2 tabs[kAdjWavesPerGroup][kAdjTab];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int lane = threadIdx.x % 64;
     const int pos = blockIdx.x * kAdjWavesPerGroup + wave;
-    const int sl = blockIdx.z;
+    const int sl0 = blockIdx.z * AS, nv = min(AS, slices - sl0);
     if (pos >= q.npos) return;
     const AdjAngle an = angles[pos];
     if (an.p < 0 || (int)blockIdx.y >= an.nb) return;
@@ -1270,39 +1284,51 @@ __global__ void __launch_bounds__(64 * kAdjWavesPerGroup) bp_adjoint_path_kernel
     const int i = blockIdx.y, mid = (an.nb - 1) / 2;
     const int b = (i & 1) ? mid + (i + 1) / 2 : mid - i / 2;
     const int p = an.p;
-    float* rec = partial + (((long)sl * q.P + p) * q.nbmax + b) * kAdjRec;
     const int J0 = an.J0base + 64 * b;
     // this lane's bins a_lo .. a_lo + 2
     const double cAd = (double)an.cA * (double)q.scale;
     const double base = ((double)(J0 + lane) - q.c0) * cAd;
     const int a_lo = (int)floor(base - 0.5 * fabs(cAd));
     const int2 rr = bands[p * q.nbmax + b];
-    const float* img = (an.cls ? B : A) + (long)sl * q.img_stride;
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(img), (short)0, (int)(q.img_stride * 4), 0x00020000);
+    // images past the pass's last read the last one again and write nothing
+    __amdgpu_buffer_rsrc_t rsrc[AS];
+#pragma unroll
+    for (int k = 0; k < AS; k++) {
+        const float* img = (an.cls ? B : A) + (long)(sl0 + min(k, nv - 1)) * q.img_stride;
+        rsrc[k] = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(img), (short)0, (int)(q.img_stride * 4),
+                                                    0x00020000);
+    }
     const int2* rt = rowtab + (long)p * q.N;
     const float* Tp = Ttab + (long)p * q.tilesX * q.tilesY;
     const float c = cosT[p], s = sinT[p], A1 = (float)(a_lo + 1);
-    float g0 = 0.0f, gm = 0.0f, ga = 0.0f;
+    float g0[AS], gm[AS], ga[AS];
+#pragma unroll
+    for (int k = 0; k < AS; k++) g0[k] = gm[k] = ga[k] = 0.0f;
     if (an.cls)
-        adj_path<1>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
+        adj_path<1, AS>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
     else
-        adj_path<0>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
-    // the path's three contributions (t-bins a_lo, a_lo + 1, a_lo + 2):
-    // max(-m, 0) = (|m| - m) / 2, 1 - |m|, max(m, 0) = (|m| + m) / 2
-    const float c0v = 0.5f * (ga - gm), c1v = g0 - ga, c2v = 0.5f * (ga + gm);
+        adj_path<0, AS>(rsrc, rt, coord, Tp, q, tabs[wave], J0, lane, rr.x, rr.y, c, s, A1, g0, gm, ga);
     // window of the band: the lanes' a_lo are monotone in the lane index
     const int kstart = min(__builtin_amdgcn_readfirstlane(a_lo), __builtin_amdgcn_readlane(a_lo, 63));
-    float* w = win[wave];
-    for (int i = lane; i < kAdjWin; i += 64) w[i] = 0.0f;
-    wave_lds_sync();
     const int wi = min(max(a_lo - kstart, 0), kAdjWin - 3);
-    atomicAdd(&w[wi], c0v);
-    atomicAdd(&w[wi + 1], c1v);
-    atomicAdd(&w[wi + 2], c2v);
-    wave_lds_sync();
-    if (lane == 0) rec[0] = __int_as_float(kstart);
-    for (int i = lane; i < kAdjWin; i += 64) rec[1 + i] = w[i];
+    float* w = win[wave];
+#pragma unroll
+    for (int k = 0; k < AS; k++) {
+        if (k >= nv) break;
+        // the path's three contributions (t-bins a_lo, a_lo + 1, a_lo + 2):
+        // max(-m, 0) = (|m| - m) / 2, 1 - |m|, max(m, 0) = (|m| + m) / 2
+        const float c0v = 0.5f * (ga[k] - gm[k]), c1v = g0[k] - ga[k], c2v = 0.5f * (ga[k] + gm[k]);
+        float* rec = partial + (((long)(sl0 + k) * q.P + p) * q.nbmax + b) * kAdjRec;
+        for (int j = lane; j < kAdjWin; j += 64) w[j] = 0.0f;
+        wave_lds_sync();
+        atomicAdd(&w[wi], c0v);
+        atomicAdd(&w[wi + 1], c1v);
+        atomicAdd(&w[wi + 2], c2v);
+        wave_lds_sync();
+        if (lane == 0) rec[0] = __int_as_float(kstart);
+        for (int j = lane; j < kAdjWin; j += 64) rec[1 + j] = w[j];
+        wave_lds_sync();   // the window is read before the next image clears it
+    }
 }
 
 // sino[s][a][k] = outmul * sum over the angle's bands (in band order) of the
@@ -1480,7 +1506,8 @@ struct risa_bp_s {
 
 namespace risa {
 // slices per pass of the sheared-path adjoint through its buffers
-constexpr int kAdjSlices = 8;
+constexpr int kAdjSlices = 16;
+constexpr int kAdjPathSlices = 2;   // images per wave of the path kernel (a divisor of kAdjSlices)
 }  // namespace risa
 
 namespace risa {
@@ -1859,10 +1886,11 @@ static void adjoint_run(risa_bp_t h, const float* img, int slices, float* sino, 
                            img + (size_t)s0 * N * N, h->adj_imgA.ptr, h->adj_imgB.ptr, N, q.PW, h->fuse_mask ? 1 : 0,
                            h->lut.center, q.img_stride);
         check_launch("adj_pad_kernel");
-        hipLaunchKernelGGL(bp_adjoint_path_kernel, dim3(h->adj_groups, q.nbmax, n), dim3(64 * kAdjWavesPerGroup), 0,
+        hipLaunchKernelGGL(bp_adjoint_path_kernel<kAdjPathSlices>,
+                           dim3(h->adj_groups, q.nbmax, ceil_div(n, kAdjPathSlices)), dim3(64 * kAdjWavesPerGroup), 0,
                            s, h->adj_imgA.ptr, h->adj_imgB.ptr, h->adj_partial.ptr, h->adj_angles.ptr,
                            h->adj_bands.ptr, h->adj_rows.ptr, h->adj_coord.ptr, h->adj_T.ptr, h->cosT.ptr, h->sinT.ptr,
-                           q);
+                           n, q);
         check_launch("bp_adjoint_path_kernel");
         hipLaunchKernelGGL(adj_reduce_kernel, dim3(P, n), dim3(256), (size_t)q.nbmax * kAdjRec * sizeof(float), s,
                            h->adj_partial.ptr, h->adj_nb.ptr, sino + (size_t)s0 * P * D, q);
