@@ -391,6 +391,40 @@ def test_masking_stage(risa, orc, geom_c0, cuda):
             assert np.array_equal(got, ref)
 
 
+def test_chain_plain_lane_groups_equal_single_frames(risa, geom_c0, frames_c0, cuda):
+    """A plain chain call of 37 frames (4 lanes x groups of up to 8 frames,
+    the last lane step a partial group, backprojected with multi-slice
+    tiles) gives every frame bit for bit what a one-frame chain with the same
+    tile variant gives it, eager and as a replayed graph."""
+    import torch
+    from risa_amd import device
+
+    g, fs = geom_c0, frames_c0
+    F = 37
+    idx = np.arange(F) % len(fs.planes)
+    raw = _t(fs.raw[idx], cuda)
+    planes = fs.planes[idx].astype(np.int32)
+    a = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_PLAIN, max_frames=F)
+    b = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_PLAIN, max_frames=1)
+    b.variant = a.variant
+    print(f"case: chain plain C0, {F} frames in lane groups, variant {a.variant & 0xFF}", flush=True)
+    img = torch.full((F,) + g.img_shape, float("nan"), dtype=torch.float32, device=cuda)
+    a(raw, planes, img)
+    one = torch.empty((1,) + g.img_shape, dtype=torch.float32, device=cuda)
+    for f in range(F):
+        b(raw[f:f + 1], planes[f:f + 1], one)
+        _sync()
+        assert torch.equal(one[0], img[f]), f"frame {f}"
+    img2 = torch.full_like(img, float("nan"))
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a(raw, planes, img2, use_graph=True)
+    s.synchronize()
+    assert torch.equal(img2, img)
+    a.close()
+    b.close()
+
+
 def test_chain_plain_matches_fp64_oracle(risa, orc, geom_c0, frames_c0, cuda):
     import torch
     from risa_amd import device
