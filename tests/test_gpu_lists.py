@@ -23,14 +23,16 @@ def _sync():
     torch.cuda.synchronize()
 
 
-def test_attenuation_list_equals_batch(risa, geom_c1, cuda):
+@pytest.mark.parametrize("F", [7, 70])
+def test_attenuation_list_equals_batch(risa, geom_c1, cuda, F):
+    """(70 frames: two launches of the 64-frame table.)"""
     import torch
     from risa_amd import device
 
-    print("case: attenuation list C1, 7 frames over 2 planes", flush=True)
+    print(f"case: attenuation list C1, {F} frames over 2 planes", flush=True)
     g = geom_c1
     rng = np.random.default_rng(3)
-    planes = np.array([1, 0, 0, 1, 1, 0, 1], np.int32)
+    planes = ((np.arange(F) * 7) // 3 % 2).astype(np.int32)
     ref = rng.uniform(20000, 60000, (g.numberOfPlanes,) + g.fan_shape).astype(np.float32)
     dark = rng.uniform(100, 300, (g.numberOfPlanes, g.numberOfFanDetectors)).astype(np.float32)
     att = device.Attenuation(g, ref, dark)
@@ -46,14 +48,14 @@ def test_attenuation_list_equals_batch(risa, geom_c1, cuda):
     att.close()
 
 
-@pytest.mark.parametrize("D", [512, 511])
-def test_fan2para_list_equals_batch(risa, geom_c1, cuda, D):
+@pytest.mark.parametrize("D,F", [(512, 9), (511, 9), (512, 70)])
+def test_fan2para_list_equals_batch(risa, geom_c1, cuda, D, F):
     import torch
     from risa_amd import device
 
-    print(f"case: fan2para list C1 D={D}, 9 frames over 2 planes", flush=True)
+    print(f"case: fan2para list C1 D={D}, {F} frames over 2 planes", flush=True)
     g = geom_c1.replace(numberOfParallelDetectors=D)
-    planes = np.array([0, 1, 1, 0, 1, 0, 0, 1, 0], np.int32)
+    planes = ((np.arange(F) * 5) // 2 % 2).astype(np.int32)
     rng = np.random.default_rng(17)
     fan = torch.from_numpy(rng.standard_normal((len(planes),) + g.fan_shape).astype(np.float32)).to(cuda)
     f2p = device.Fan2Para(g)
@@ -67,31 +69,32 @@ def test_fan2para_list_equals_batch(risa, geom_c1, cuda, D):
     f2p.close()
 
 
-@pytest.mark.parametrize("P,D", [(1000, 512), (999, 512), (360, 500), (361, 261)])
-def test_filter_list_equals_batch(risa, geom_c0, cuda, P, D):
+@pytest.mark.parametrize("P,D,S", [(1000, 512, 5), (999, 512, 5), (360, 500, 5), (361, 261, 5), (101, 512, 70)])
+def test_filter_list_equals_batch(risa, geom_c0, cuda, P, D, S):
     import torch
     from risa_amd import device
 
-    print(f"case: filter list P={P} D={D}, 5 sinograms", flush=True)
+    print(f"case: filter list P={P} D={D}, {S} sinograms", flush=True)
     g = geom_c0.replace(numberOfParallelProjections=P, numberOfParallelDetectors=D)
     rng = np.random.default_rng(P + D)
-    x = torch.from_numpy(rng.standard_normal((5, P, D)).astype(np.float32)).to(cuda)
+    x = torch.from_numpy(rng.standard_normal((S, P, D)).astype(np.float32)).to(cuda)
     filt = device.Filter(g)
     sinos = _frames(x)
-    filt(x, 5)
+    filt(x, S)
     filt.run_list(sinos)
     _sync()
-    for f in range(5):
+    for f in range(S):
         assert torch.equal(sinos[f], x[f]), f"sinogram {f}"
     filt.close()
 
 
-@pytest.mark.parametrize("S", [7, 16])
-def test_backprojection_list_equals_batch_every_variant(risa, geom_c0, cuda, S):
+@pytest.mark.parametrize("S,N", [(7, 256), (16, 256), (70, 64)])
+def test_backprojection_list_equals_batch_every_variant(risa, geom_c0, cuda, S, N):
+    """(70 slices: two launches of the 64-slice table, the second a partial group.)"""
     import torch
     from risa_amd import device
 
-    g = geom_c0.replace(numberOfPixels=256, numberOfParallelProjections=180, numberOfParallelDetectors=200)
+    g = geom_c0.replace(numberOfPixels=N, numberOfParallelProjections=180, numberOfParallelDetectors=200)
     N, P, D = g.numberOfPixels, g.numberOfParallelProjections, g.numberOfParallelDetectors
     rng = np.random.default_rng(S)
     sino = torch.from_numpy(rng.standard_normal((S, P, D)).astype(np.float32)).to(cuda)
@@ -103,6 +106,12 @@ def test_backprojection_list_equals_batch_every_variant(risa, geom_c0, cuda, S):
         bp.variant = v
         imgb = torch.empty((S, N, N), dtype=torch.float32, device=cuda)
         bp.batch(sino, S, imgb)
+        if S > 64:   # the contiguous batch against one-slice launches too
+            one = torch.empty((N, N), dtype=torch.float32, device=cuda)
+            for k in (0, 63, 64, S - 1):
+                bp(sino[k], one)
+                _sync()
+                assert torch.equal(one, imgb[k]), f"variant {v} slice {k} (batch vs single)"
         imgs = [torch.full((N, N), float("nan"), dtype=torch.float32, device=cuda) for _ in range(S)]
         bp.run_list(sinos, imgs)
         _sync()
