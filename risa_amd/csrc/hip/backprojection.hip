@@ -41,6 +41,7 @@ struct BpParams {
     int AE;              // angles per staged chunk
     int zx, zy;          // log2 of the tile grid's width / height for the XCD-aware order, or -1
     int xg;              // slice groups dealt to XCDs whole (bp_work), set per launch
+    const int* order;    // bp_work's dispatch order tables (4 of tilesX * tilesY ranks) or null
 };
 
 // The tile of workgroup b.  Workgroups are dealt round-robin to the chip's 8
@@ -74,7 +75,12 @@ __device__ __forceinline__ int2 bp_tile_xy(int b, const BpParams& p)
 // share a group, each a contiguous range of its tile order.  One XCD's L2
 // then holds few sinograms and each is fetched from HBM about once, where
 // splitting every slice's tiles over all 8 XCDs fetches it up to 8 times.
-// Otherwise the tile order of bp_tile_xy and group blockIdx.y.
+// Otherwise the tile order of bp_tile_xy and group blockIdx.y.  Within an
+// XCD's range the tiles are dispatched heaviest first when p.order is set
+// (table lm = log2 of the XCDs per group: each range of T / 2^lm ranks
+// sorted by the tile's pixels inside the mask), so the workgroups that end
+// a launch are the light edge tiles, not the full ones; L2 locality does not
+// depend on the order inside a range.
 __device__ __forceinline__ int bp_work(const BpParams& p, int2& txy)
 {
     if (!p.xg) {
@@ -86,11 +92,14 @@ __device__ __forceinline__ int bp_work(const BpParams& p, int2& txy)
     const int xcd = (int)(L & 7);
     const long k = L >> 3;
     if (G >= 8) {
-        txy = bp_tile_of_rank((int)(k % T), p);
+        const int rank = (int)(k % T);
+        txy = bp_tile_of_rank(p.order ? p.order[rank] : rank, p);
         return xcd + 8 * (int)(k / T);
     }
     const int m = 8 / G;   // XCDs per group
-    txy = bp_tile_of_rank((xcd / G) * (T / m) + (int)k, p);
+    const int lm = m == 2 ? 1 : (m == 4 ? 2 : (m == 8 ? 3 : 0));
+    const int rank = (xcd / G) * (T / m) + (int)k;
+    txy = bp_tile_of_rank(p.order ? p.order[lm * T + rank] : rank, p);
     return xcd % G;
 }
 // The host's switch for bp_work.
@@ -1502,6 +1511,10 @@ struct risa_bp_s {
     risa::DeviceBuffer<float> adj_partial;   // [kAdjSlices][P][nbmax][kAdjRec]
     risa::DeviceBuffer<float> adj_T;         // [P][tilesY][tilesX] the forward's tile origins (variant in use)
     risa::DeviceBuffer<float> adj_coord;     // [N] the forward's pixel coordinates
+    // bp_work's heaviest-first dispatch tables per tile shape (FP32 variants,
+    // then FP16 variants): offset into tile_order, or -1
+    risa::DeviceBuffer<int> tile_order;
+    std::vector<long> order_off;
 };
 
 namespace risa {
@@ -1537,6 +1550,49 @@ static void set_tile_order(BpParams& p, int tilesY)
     p.zx = log2_exact(p.tilesX);
     p.zy = log2_exact(tilesY);
     if (p.zx < 0 || p.zy < 0 || p.zx + p.zy < 3) p.zx = p.zy = -1;
+}
+
+// bp_work's dispatch tables for a TX x TY tile grid (bp_tile_of_rank's rank
+// order, host side): table lm holds, for each range of T / 2^lm consecutive
+// ranks, the range's ranks sorted by the tile's pixels inside the image (and
+// the mask), most first (ties in rank order).  Empty when T is not a
+// multiple of 8 (bp_work then keeps the plain order).
+static std::vector<int> dispatch_order(const BpParams& p, int TX, int TY, bool mask)
+{
+    const int tilesY = ceil_div(p.N, TY), T = p.tilesX * tilesY;
+    std::vector<int> out;
+    if (T % 8 != 0) return out;
+    std::vector<long> cost((size_t)T);
+    for (int r = 0; r < T; r++) {
+        int tx = 0, ty = 0;
+        if (p.zx < 0) {
+            tx = r % p.tilesX;
+            ty = r / p.tilesX;
+        } else {
+            int bit = 0;
+            for (int i = 0; i < p.zx || i < p.zy; i++) {
+                if (i < p.zx) tx |= ((r >> bit++) & 1) << i;
+                if (i < p.zy) ty |= ((r >> bit++) & 1) << i;
+            }
+        }
+        long c = 0;
+        for (int y = ty * TY; y < std::min(p.N, ty * TY + TY); y++)
+            for (int x = tx * TX; x < std::min(p.N, tx * TX + TX); x++) {
+                const double dx = (double)((float)x - p.center), dy = (double)((float)y - p.center);
+                if (!mask || dx * dx + dy * dy <= (double)p.N * p.N * 0.25) c++;
+            }
+        cost[(size_t)r] = c;
+    }
+    out.resize((size_t)4 * T);
+    for (int lm = 0; lm < 4; lm++) {
+        const int block = T >> lm;
+        for (int b0 = 0; b0 < T; b0 += block) {
+            int* o = out.data() + (size_t)lm * T + b0;
+            for (int i = 0; i < block; i++) o[i] = b0 + i;
+            std::stable_sort(o, o + block, [&](int a, int b) { return cost[(size_t)a] > cost[(size_t)b]; });
+        }
+    }
+    return out;
 }
 
 static BpParams make_params(const risa_bp_s* h, int tx, int ty)
@@ -1686,6 +1742,19 @@ extern "C" int risa_hip_bp_create(const risa_geom* g, int device, risa_bp_t* out
             h->x4.resize((size_t)kF16StageGroups * plane);
             setup_adjoint(h);
             h->imgT.resize((size_t)g->numberOfPixels * g->numberOfPixels);   // the gather adjoint's transpose
+            // heaviest-first dispatch tables of every tile shape
+            std::vector<int> orders;
+            for (int v = 0; v < kNumVariants + kNumF16Variants; v++) {
+                const int tx = v < kNumVariants ? kVariants[v].tx : kF16Variants[v - kNumVariants].tx;
+                const int ty = v < kNumVariants ? kVariants[v].ty : kF16Variants[v - kNumVariants].ty;
+                const std::vector<int> o = dispatch_order(make_params(h, tx, ty), tx, ty, h->fuse_mask);
+                h->order_off.push_back(o.empty() ? -1L : (long)orders.size());
+                orders.insert(orders.end(), o.begin(), o.end());
+            }
+            if (!orders.empty()) {
+                h->tile_order.resize(orders.size());
+                h->tile_order.upload(orders);
+            }
         } catch (...) {
             delete h;
             throw;
@@ -1716,6 +1785,7 @@ static void launch_linear_at(risa_bp_t h, int variant, SinoAt sino, ImgAt img, i
         }
         dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(n, v.sb));
         p.xg = xcd_groups(grid.x, grid.y);
+        p.order = p.xg && h->order_off[(size_t)variant] >= 0 ? h->tile_order.ptr + h->order_off[(size_t)variant] : nullptr;
         hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, L, h->cosT.ptr, h->sinT.ptr, n, p);
         check_launch("bp_linear_kernel");
     }
@@ -1784,6 +1854,8 @@ static void launch_f16x4(risa_bp_t h, int variant, const uint2* sino, float in_s
     if (lds > kMaxLds) fail(RISA_ERR_UNSUPPORTED, "backprojection tile needs %zu B of LDS", lds);
     dim3 grid(p.tilesX * ceil_div(p.N, v.ty), ceil_div(ceil_div(slices, kF16Group), v.g));
     p.xg = xcd_groups(grid.x, grid.y);
+    const long off = h->order_off[(size_t)(kNumVariants + variant)];
+    p.order = p.xg && off >= 0 ? h->tile_order.ptr + off : nullptr;
     hipLaunchKernelGGL(v.fn, grid, dim3(threads), lds, s, sino, img, h->cosT.ptr, h->sinT.ptr, slices, p);
     check_launch("bp_linear_f16_kernel");
 }
