@@ -35,7 +35,7 @@ METRIC = "512² slices/sec on 1 GPU (mixed vs plain FP32) at stated RMSE; BP HBM
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_UPDATE = 10          # t: mul+fma; two weights; two accumulate FMAs
-BP_SLICES_IN_FLIGHT = 32      # the plain chain keeps 4 lanes x 8 frames in flight
+BP_SLICES_IN_FLIGHT = 32      # the plain chain keeps 2 lanes x 16 frames in flight
 
 
 def log(*a):
@@ -297,7 +297,7 @@ def main():
     img_mixed_sample = img[:planes].cpu().numpy()
 
     # Dominant kernel: backprojection, timed with HIP events on its stream,
-    # at the concurrency the plain chain runs it (4 lanes x 8 frames in
+    # at the concurrency the plain chain runs it (2 lanes x 16 frames in
     # flight = one launch of 32 slices here, tile shape autotuned for it).
     P, D = g.numberOfParallelProjections, g.numberOfParallelDetectors
     S = BP_SLICES_IN_FLIGHT

@@ -413,7 +413,7 @@ int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float*
 int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
                        float* img, int use_graph, void* stream);
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
- * 2 filtered (PLAIN: FP32, eight frame slots per lane, frame 0 of a run in
+ * 2 filtered (PLAIN: FP32, sixteen frame slots per lane, frame 0 of a run in
  * slot 0; MIXED: FP16 4-slice layout, one slot per frame). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces

@@ -6,7 +6,7 @@
 // stream-ordered, and a captured hipGraph replays the launch sequence.
 //
 // PLAIN: frame by frame, FP32 throughout (FFT filter, FP32 backprojector).
-//        Groups of `per` consecutive frames (up to 8; each stage runs once
+//        Groups of `per` consecutive frames (up to 16; each stage runs once
 //        per group) are dealt round-robin to `lanes` streams (own scratch
 //        buffers each) forked from and joined back into the caller's stream,
 //        so lanes x per frames are in flight -- the way the reference
@@ -24,8 +24,8 @@ int risa_filter_run_oop(risa_filter_t h, const float* in, float* out, long rows,
 
 namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
-constexpr int kPlainLanes = 4;            // streams in PLAIN mode
-constexpr int kPlainMaxPer = 8;           // frames per lane step in PLAIN mode, at most
+constexpr int kPlainLanes = 2;            // streams in PLAIN mode
+constexpr int kPlainMaxPer = 16;          // frames per lane step in PLAIN mode, at most
 }  // namespace
 
 // Graph executables a chain keeps (one per buffer set and call shape); the

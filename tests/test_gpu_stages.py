@@ -392,7 +392,7 @@ def test_masking_stage(risa, orc, geom_c0, cuda):
 
 
 def test_chain_plain_lane_groups_equal_single_frames(risa, geom_c0, frames_c0, cuda):
-    """A plain chain call of 37 frames (4 lanes x groups of up to 8 frames,
+    """A plain chain call of 37 frames (2 lanes x groups of up to 16 frames,
     the last lane step a partial group, backprojected with multi-slice
     tiles) gives every frame bit for bit what a one-frame chain with the same
     tile variant gives it, eager and as a replayed graph."""
