@@ -255,7 +255,11 @@ int risa_hip_fan2para_destroy(risa_fan2para_t h);
 
 /* ----------------------------------------------------------------- Filter -- */
 /* Replaces Filter::Filter + initCuFFT + designFilter (Filter.cu:44-65,
- * 145-191).  mode: enum risa_filter_mode. */
+ * 145-191).  mode: enum risa_filter_mode.  numberOfParallelDetectors may be
+ * any count from 1 to 16384 in FFT32 mode (counts whose factors are not all
+ * 2, 3 and 5 are filtered over a zero-padded length >= 2D - 1, which gives
+ * the same circular convolution), up to 8192 in GEMM mode; larger counts
+ * are refused with RISA_ERR_UNSUPPORTED.  The tap gradient takes D <= 2048. */
 int risa_hip_filter_create(const risa_geom* g, int device, int mode, risa_filter_t* out);
 /* Replace the D/2+1 taps (host array); used for learnable taps. */
 int risa_hip_filter_set_taps(risa_filter_t h, const float* taps);
@@ -288,7 +292,7 @@ int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const float* g, i
  * instead of the handle's buffers: the call allocates nothing (for callers
  * that cannot allocate: code under stream capture, tight serving loops).
  * No counterpart in the reference.  The array (8.4 MB per GPU, held until
- * the process ends) fits any filter a handle accepts (D <= 2048); larger
+ * the process ends) fits any filter the tap gradient takes (D <= 2048); larger
  * needs are refused with RISA_ERR_UNSUPPORTED.  All calls on one GPU share
  * it: calls that overlap on one GPU (different streams, no ordering between
  * them) race and give wrong results. */

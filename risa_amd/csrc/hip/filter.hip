@@ -103,6 +103,26 @@ __device__ __forceinline__ void dft8(float2* a)
     a[7] = csub(e[3], o3);
 }
 
+// Butterfly b (0 <= b < m = D / r) of one radix-r Stockham pass over a row of
+// length D (src -> dst; Ns = product of the earlier passes' radices,
+// stride = D / (Ns r)).
+template <int S>
+__device__ __forceinline__ void stockham_butterfly(const float2* src, float2* dst, int r, int Ns, int m, int stride,
+                                                   const float2* __restrict__ tw, int b)
+{
+    const int k = b % Ns;
+    float2 a[5];
+    a[0] = src[b];
+    for (int q = 1; q < r; q++) {
+        float2 w = tw[q * k * stride];
+        if (S > 0) w.y = -w.y;
+        a[q] = cmul(src[b + q * m], w);
+    }
+    butterfly<S>(a, r);
+    const int base = (b - k) * r + k;
+    for (int q = 0; q < r; q++) dst[base + q * Ns] = a[q];
+}
+
 // One Stockham pass over a row held in LDS (src -> dst), threads t0..T-1.
 template <int S>
 __device__ __forceinline__ void stockham_pass(const float2* src, float2* dst, int D, int r, int Ns,
@@ -110,19 +130,7 @@ __device__ __forceinline__ void stockham_pass(const float2* src, float2* dst, in
 {
     const int m = D / r;
     const int stride = D / (Ns * r);
-    for (int b = t0; b < m; b += T) {
-        const int k = b % Ns;
-        float2 a[5];
-        a[0] = src[b];
-        for (int q = 1; q < r; q++) {
-            float2 w = tw[q * k * stride];
-            if (S > 0) w.y = -w.y;
-            a[q] = cmul(src[b + q * m], w);
-        }
-        butterfly<S>(a, r);
-        const int base = (b - k) * r + k;
-        for (int q = 0; q < r; q++) dst[base + q * Ns] = a[q];
-    }
+    for (int b = t0; b < m; b += T) stockham_butterfly<S>(src, dst, r, Ns, m, stride, tw, b);
 }
 
 // The rows of one FFT-filter launch: row r is row r % per of frame r / per,
@@ -196,6 +204,78 @@ __global__ void __launch_bounds__(256) fft_filter_kernel(RowFrames fr, uint16_t*
         if (out16) {
             out16[ra * D + n] = float_to_half_bits(v.x * out_scale);
             if (hasb) out16[rb * D + n] = float_to_half_bits(v.y * out_scale);
+        }
+    }
+}
+
+// Transform lengths too long for one workgroup's LDS: the same row-pair
+// filter with each Stockham pass a launch over a chunk of row pairs held in
+// global memory, buf[y][0..M) for pair pair0 + y (grid y = pairs in the
+// chunk, every one of them < (rows + 1) / 2).  Same butterflies, twiddles and
+// response as fft_filter_kernel, so both paths give the same values.
+struct PairRows {
+    long ra, rb;
+    int fa, fb;
+    long qa, qb;
+    bool hasb;
+};
+
+__device__ __forceinline__ PairRows pair_rows(const RowFrames& fr, long pair, long rows)
+{
+    PairRows p;
+    p.ra = pair * 2;
+    p.rb = p.ra + 1;
+    p.hasb = p.rb < rows;
+    p.fa = (int)(p.ra / fr.per);
+    p.fb = p.hasb ? (int)(p.rb / fr.per) : p.fa;
+    p.qa = p.ra - (long)p.fa * fr.per;
+    p.qb = p.rb - (long)p.fb * fr.per;
+    return p;
+}
+
+__global__ void __launch_bounds__(256) fftg_load_kernel(RowFrames fr, float2* __restrict__ buf, int D, int M,
+                                                        long rows, long pair0)
+{
+    const PairRows p = pair_rows(fr, pair0 + blockIdx.y, rows);
+    const float* xa = fr.in[p.fa] + p.qa * D;
+    const float* xb = fr.in[p.fb] + p.qb * D;
+    float2* b = buf + (size_t)blockIdx.y * M;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < M; n += gridDim.x * blockDim.x)
+        b[n] = n < D ? make_float2(xa[n], p.hasb ? xb[n] : 0.0f) : make_float2(0.0f, 0.0f);
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) fftg_pass_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                        int M, int r, int Ns, const float2* __restrict__ tw)
+{
+    const int m = M / r;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m) return;
+    const size_t off = (size_t)blockIdx.y * M;
+    stockham_butterfly<S>(src + off, dst + off, r, Ns, m, M / (Ns * r), tw, b);
+}
+
+__global__ void __launch_bounds__(256) fftg_scale_kernel(float2* __restrict__ buf, const float* __restrict__ H, int M)
+{
+    float2* b = buf + (size_t)blockIdx.y * M;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < M; n += gridDim.x * blockDim.x) b[n] = cscale(b[n], H[n]);
+}
+
+__global__ void __launch_bounds__(256) fftg_store_kernel(RowFrames fr, const float2* __restrict__ buf,
+                                                         uint16_t* __restrict__ out16, float out_scale, int D, int M,
+                                                         long rows, long pair0)
+{
+    const PairRows p = pair_rows(fr, pair0 + blockIdx.y, rows);
+    const float2* b = buf + (size_t)blockIdx.y * M;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < D; n += gridDim.x * blockDim.x) {
+        const float2 v = b[n];
+        if (fr.has_out) {
+            fr.out[p.fa][p.qa * D + n] = v.x;
+            if (p.hasb) fr.out[p.fb][p.qb * D + n] = v.y;
+        }
+        if (out16) {
+            out16[p.ra * D + n] = float_to_half_bits(v.x * out_scale);
+            if (p.hasb) out16[p.rb * D + n] = float_to_half_bits(v.y * out_scale);
         }
     }
 }
@@ -896,6 +976,8 @@ struct risa_filter_s {
     risa::DeviceBuffer<double2> xsum;     // padded grad: their sum [M/2+1]
     risa::DeviceBuffer<double> xcorr;     // padded grad: wrapped cross-correlation [D]
     risa::DeviceBuffer<float> stage;      // GEMM-mode in-place staging [kGemmStageRows][D]
+    risa::DeviceBuffer<float2> gbuf;      // M > kMaxLdsFftLen: two buffers of [gchunk][M]
+    long gchunk = 0;                      // row pairs per global-memory chunk
 };
 
 using namespace risa;
@@ -914,14 +996,23 @@ static bool make_plan(int D, FftPlan& p)
 
 // Circulant coefficients of the taps, c[d] = h0 + 2 sum_k h_k cos(2 pi k d / D)
 // + [D even] h_{D/2} (-1)^d, in double.
+// cos(2 pi n / L) for n = 0 .. L-1, each evaluated as the loops below would.
+static std::vector<double> cos_table(int L)
+{
+    std::vector<double> t(L);
+    for (int n = 0; n < L; n++) t[n] = std::cos(2.0 * M_PI * (double)n / L);
+    return t;
+}
+
 static std::vector<double> circulant(const std::vector<float>& taps, int D)
 {
     std::vector<double> c(D);
+    const std::vector<double> cs = cos_table(D);
     const int half = D / 2;
     for (int d = 0; d < D; d++) {
         double acc = taps[0];
         const int kmax = (D % 2 == 0) ? half - 1 : half;
-        for (int k = 1; k <= kmax; k++) acc += 2.0 * taps[k] * std::cos(2.0 * M_PI * (double)(((long)k * d) % D) / D);
+        for (int k = 1; k <= kmax; k++) acc += 2.0 * taps[k] * cs[((long)k * d) % D];
         if (D % 2 == 0) acc += taps[half] * ((d % 2) ? -1.0 : 1.0);
         c[d] = acc;
     }
@@ -942,9 +1033,10 @@ static void upload_response(risa_filter_s* h)
         // -(D-1) .. D-1, zeros elsewhere), with the 1/M of the unnormalised
         // inverse transform
         c = circulant(h->taps, D);
+        const std::vector<double> cs = cos_table(M);
         for (int k = 0; k < M; k++) {
             double acc = c[0];
-            for (int j = 1; j < D; j++) acc += 2.0 * c[j] * std::cos(2.0 * M_PI * (double)(((long)j * k) % M) / M);
+            for (int j = 1; j < D; j++) acc += 2.0 * c[j] * cs[((long)j * k) % M];
             Hf[k] = (float)(acc / M);
         }
     }
@@ -978,10 +1070,18 @@ static void upload_response(risa_filter_s* h)
     }
 }
 
-// Detector counts up to kMaxDetectors are filtered; the FFT a workgroup holds
-// in LDS (one row pair, two buffers) is at most kMaxFftLen long.
-constexpr int kMaxDetectors = 2048;
-constexpr int kMaxFftLen = 4096;
+// Detector counts up to kMaxDetectors are filtered (the bound keeps the
+// host's O(M D) response set-up of a padded length to about a second).
+// Transforms up to kMaxLdsFftLen run in one workgroup's LDS (one row pair,
+// two buffers); longer ones pass through global memory in chunks of row
+// pairs whose two buffers take at most kGlobalFftBytes.  The tap gradient's
+// LDS kernels take D <= kMaxGradDetectors, M <= kMaxGradFftLen.
+constexpr int kMaxDetectors = 16384;
+constexpr int kMaxLdsFftLen = 4096;
+constexpr size_t kGlobalFftBytes = (size_t)128 << 20;
+constexpr int kMaxGradDetectors = 2048;
+constexpr int kMaxGradFftLen = 4096;
+constexpr int kMaxGemmDetectors = 8192;   // GEMM mode: D x D split circulant
 // Rows per GEMM-mode in-place pass through the staging buffer.
 constexpr long kGemmStageRows = 16384;
 
@@ -991,9 +1091,8 @@ static int fft_length(int D, FftPlan& plan)
 {
     if (D < 1 || D > kMaxDetectors) return 0;
     if (make_plan(D, plan)) return D;
-    for (int M = 2 * D - 1; M <= kMaxFftLen; M++)
+    for (int M = 2 * D - 1;; M++)   // ends at the next power of two at the latest
         if (make_plan(M, plan)) return M;
-    return 0;
 }
 
 static void resolve_scratch(int device);
@@ -1015,7 +1114,14 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
             h->M = fft_length(D, h->plan);
             if (h->M == 0)
                 fail(RISA_ERR_UNSUPPORTED, "%d detectors: the filter takes 1 .. %d", D, kMaxDetectors);
+            if (mode == RISA_FILTER_MODE_GEMM_SPLIT && D > kMaxGemmDetectors)
+                fail(RISA_ERR_UNSUPPORTED, "%d detectors: GEMM mode takes up to %d (FFT32 mode up to %d)", D,
+                     kMaxGemmDetectors, kMaxDetectors);
             const int M = h->M;
+            if (M > kMaxLdsFftLen) {
+                h->gchunk = std::max(1L, std::min(65535L, (long)(kGlobalFftBytes / (2 * (size_t)M * sizeof(float2)))));
+                h->gbuf.resize((size_t)2 * h->gchunk * M);
+            }
             std::vector<float2> tw(M);
             for (int n = 0; n < M; n++)
                 tw[n] = make_float2((float)std::cos(-2.0 * M_PI * n / M), (float)std::sin(-2.0 * M_PI * n / M));
@@ -1033,7 +1139,9 @@ extern "C" int risa_hip_filter_create(const risa_geom* g, int device, int mode, 
                 h->stage.resize((size_t)kGemmStageRows * D);
             }
             // tap-gradient sums, sized here so run calls allocate nothing
-            if (M == D) {
+            if (D > kMaxGradDetectors || M > kMaxGradFftLen) {
+                // no tap gradient at this length (refused by grad_taps)
+            } else if (M == D) {
                 h->scratch.resize((size_t)kTapBlocks * (D / 2 + 1));
             } else {
                 h->xpart.resize((size_t)kTapBlocks * (M / 2 + 1));
@@ -1069,11 +1177,52 @@ extern "C" int risa_hip_filter_get_taps(risa_filter_t h, float* taps)
     });
 }
 
+// M > kMaxLdsFftLen: per chunk of row pairs, load (zero-padded), the forward
+// passes, the response, the inverse passes and the store, stream-ordered
+// through the handle's two buffers (chunks reuse them in order).
+static void launch_fft_global(risa_filter_t h, const RowFrames& fr, uint16_t* out16, float out_scale, long rows,
+                              hipStream_t s)
+{
+    const int D = h->g.numberOfParallelDetectors, M = h->M;
+    const long pairs = (rows + 1) / 2;
+    const unsigned row_blocks = (unsigned)std::min(ceil_div(M, 256), 64);   // grid-stride over a row
+    for (long p0 = 0; p0 < pairs; p0 += h->gchunk) {
+        const unsigned np = (unsigned)std::min(h->gchunk, pairs - p0);
+        float2* src = h->gbuf.ptr;
+        float2* dst = src + (size_t)h->gchunk * M;
+        hipLaunchKernelGGL(fftg_load_kernel, dim3(row_blocks, np), dim3(256), 0, s, fr, src, D, M, rows, p0);
+        check_launch("fftg_load_kernel");
+        for (int dir = 0; dir < 2; dir++) {
+            int Ns = 1;
+            for (int st = 0; st < h->plan.nst; st++) {
+                const int r = h->plan.radix[st];
+                const dim3 grid((unsigned)ceil_div(M / r, 256), np);
+                if (dir == 0) hipLaunchKernelGGL(fftg_pass_kernel<-1>, grid, dim3(256), 0, s, src, dst, M, r, Ns, h->tw.ptr);
+                else hipLaunchKernelGGL(fftg_pass_kernel<+1>, grid, dim3(256), 0, s, src, dst, M, r, Ns, h->tw.ptr);
+                check_launch("fftg_pass_kernel");
+                Ns *= r;
+                std::swap(src, dst);
+            }
+            if (dir == 0) {
+                hipLaunchKernelGGL(fftg_scale_kernel, dim3(row_blocks, np), dim3(256), 0, s, src, h->H.ptr, M);
+                check_launch("fftg_scale_kernel");
+            }
+        }
+        hipLaunchKernelGGL(fftg_store_kernel, dim3(row_blocks, np), dim3(256), 0, s, fr, src, out16, out_scale, D, M,
+                           rows, p0);
+        check_launch("fftg_store_kernel");
+    }
+}
+
 static void launch_fft(risa_filter_t h, const RowFrames& fr, uint16_t* out16, float out_scale, long rows,
                        hipStream_t s)
 {
     const int D = h->g.numberOfParallelDetectors, M = h->M;
     const long pairs = (rows + 1) / 2;
+    if (M > kMaxLdsFftLen) {
+        launch_fft_global(h, fr, out16, out_scale, rows, s);
+        return;
+    }
     if (D == 512 && M == 512) {
         hipLaunchKernelGGL(fft512_filter_kernel, dim3((unsigned)pairs), dim3(64), 0, s, fr, out16, out_scale,
                            h->tw.ptr, h->H.ptr, rows);
@@ -1255,6 +1404,9 @@ static void grad_taps(risa_filter_t h, const float* x, const float* g, int slice
                       hipStream_t s)
 {
     const int D = h->g.numberOfParallelDetectors, K = D / 2 + 1, M = h->M;
+    if (D > kMaxGradDetectors || M > kMaxGradFftLen)
+        fail(RISA_ERR_UNSUPPORTED, "tap gradient of %d detectors (transform length %d): it takes D <= %d, M <= %d", D,
+             M, kMaxGradDetectors, kMaxGradFftLen);
     const long rows = (long)slices * h->g.numberOfParallelProjections;
     if (M != D) {
         launch_grad_taps_padded(h, x, g, rows, kTapBlocks, dtaps, sc, s);
@@ -1283,15 +1435,15 @@ extern "C" int risa_hip_filter_grad_taps(risa_filter_t h, const float* x, const 
 // ------------------------------------------- allocation-free tap gradient --
 // The partial sums of risa_hip_filter_grad_taps_noalloc live in `scratch`, a
 // file-scope device array (one per GPU, held until the process ends) sized
-// for the largest filter a handle accepts (D <= 2048; transform length
+// for the largest filter the tap gradient takes (D <= 2048; transform length
 // M <= 4096): kTapBlocks x (M/2 + 1) complex partials, the summed spectrum
 // and the wrapped cross-correlation -- 8.4 MB.  Host code never passes the
 // array's name to a kernel: each device's address comes from
 // hipGetSymbolAddress, resolved once per device when a filter handle is
 // created there (set-up, outside any capture), and kept in scratch_ptr.
-constexpr size_t kScratchPartialBytes = (size_t)kTapBlocks * (kMaxFftLen / 2 + 1) * sizeof(float2);
-constexpr size_t kScratchSumBytes = (size_t)(kMaxFftLen / 2 + 1) * sizeof(double2);
-constexpr size_t kScratchCorrBytes = (size_t)kMaxDetectors * sizeof(double);
+constexpr size_t kScratchPartialBytes = (size_t)kTapBlocks * (kMaxGradFftLen / 2 + 1) * sizeof(float2);
+constexpr size_t kScratchSumBytes = (size_t)(kMaxGradFftLen / 2 + 1) * sizeof(double2);
+constexpr size_t kScratchCorrBytes = (size_t)kMaxGradDetectors * sizeof(double);
 constexpr size_t kScratchBytes = kScratchPartialBytes + kScratchSumBytes + kScratchCorrBytes;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -1325,7 +1477,7 @@ extern "C" int risa_hip_filter_grad_taps_noalloc(risa_filter_t h, const float* x
         const size_t need = M == D ? (size_t)kTapBlocks * (D / 2 + 1) * sizeof(float)
                                    : (size_t)kTapBlocks * (M / 2 + 1) * sizeof(float2) + (size_t)(M / 2 + 1) *
                                      sizeof(double2) + (size_t)D * sizeof(double);
-        if (D > kMaxDetectors || M > kMaxFftLen || need > kScratchBytes)
+        if (D > kMaxGradDetectors || M > kMaxGradFftLen || need > kScratchBytes)
             fail(RISA_ERR_UNSUPPORTED, "tap gradient of %d detectors needs %zu B of scratch; the static array holds %zu B",
                  D, need, kScratchBytes);
         GradScratch sc{};

@@ -51,6 +51,84 @@ def test_filter_fft32_any_length(risa, orc, geom_c0, cuda, D):
     flt.close()
 
 
+# Transforms longer than 4096 run pass by pass through global memory:
+# 2049 = 3 * 683 (padded to 4320), 4099 (prime, padded to 8640), 5000 and
+# 16384 (direct); 3000 and 4096 stay in LDS.  Three slices of 37 rows leave
+# the last row pair half empty and pairs straddling sinograms.
+LONG_D = [(2049, 37), (3000, 37), (4096, 37), (4099, 37), (5000, 37), (16384, 5)]
+
+
+@pytest.mark.parametrize("D,P", LONG_D)
+def test_filter_fft32_long_rows(risa, orc, geom_c0, cuda, D, P):
+    """FFT32 filtering of long rows against the FP64 circulant; the frame
+    list, the FP16 output and the in-place call give the same values; the
+    adjoint passes the dot test; the tap gradient refuses lengths its LDS
+    kernels do not hold."""
+    import torch
+    from risa_amd import _lib, device
+
+    print(f"case: FFT32 filter long rows D={D} P={P}", flush=True)
+    S = 3
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=P, filterType=3)
+    rng = np.random.default_rng(D + P)
+    x = (rng.standard_normal((S,) + g.par_shape) * 3.0).astype(np.float32)
+    flt = device.Filter(g, 0)
+    t = _t(x, cuda)
+    flt(t, S)
+    _sync()
+    got = t.cpu().numpy()
+    ref = orc.filter_rows_f64(orc.taps_f32(g).astype(np.float64), x.astype(np.float64))
+    err = rel_rmse(got, ref)
+    print(f"  rel RMSE vs FP64 {err:.2e}", flush=True)
+    assert err < 2e-6
+    sinos = [_t(x[k], cuda) for k in range(S)]
+    flt.run_list(sinos)
+    _sync()
+    for k in range(S):
+        assert torch.equal(sinos[k], t[k]), f"list sinogram {k}"
+    out16 = torch.empty((S,) + g.par_shape, dtype=torch.float16, device=cuda)
+    flt.f16(_t(x, cuda), out16, S, 256.0)
+    _sync()
+    assert torch.equal(out16, (t * 256.0).half())
+    y = rng.standard_normal((S,) + g.par_shape).astype(np.float32)
+    fty = _t(y, cuda)
+    flt.adjoint(fty, S)
+    _sync()
+    lhs = float(np.sum(got.astype(np.float64) * y))
+    rhs = float(np.sum(x.astype(np.float64) * fty.cpu().numpy()))
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"  dot test rel gap {gap:.2e}", flush=True)
+    assert gap <= 1e-5
+    if D > 2048:
+        dt = torch.empty(D // 2 + 1, dtype=torch.float32, device=cuda)
+        with pytest.raises(_lib.RisaError, match="RISA_ERR_UNSUPPORTED"):
+            flt.grad_taps(t, t, S, dt)
+    flt.close()
+
+
+def test_filter_fft32_long_rows_many_chunks(risa, orc, geom_c0, cuda):
+    """1100 rows of 16384 detectors are 550 row pairs, filtered in two chunks
+    of the global-memory transform (512 pairs each at this length); rows on
+    both sides of the chunk boundary and the last row against the FP64
+    circulant."""
+    from risa_amd import device
+
+    D, P = 16384, 1100
+    print(f"case: FFT32 filter D={D}, {P} rows in two chunks", flush=True)
+    g = geom_c0.replace(numberOfParallelDetectors=D, numberOfParallelProjections=P, filterType=1)
+    x = (np.random.default_rng(11).standard_normal((P, D)) * 2.0).astype(np.float32)
+    flt = device.Filter(g, 0)
+    t = _t(x, cuda)
+    flt(t, 1)
+    _sync()
+    rows = [0, 1, 1022, 1023, 1024, 1025, P - 1]
+    ref = orc.filter_rows_f64(orc.taps_f32(g).astype(np.float64), x[rows].astype(np.float64))
+    err = rel_rmse(t.cpu().numpy()[rows], ref)
+    print(f"  rel RMSE vs FP64 {err:.2e}", flush=True)
+    assert err < 2e-6
+    flt.close()
+
+
 @pytest.mark.parametrize("D", [7, 257, 261, 293, 303, 1021])
 def test_filter_gemm_and_adjoint_any_length(risa, orc, geom_c0, cuda, D):
     """GEMM mode (no FFT length condition) in place against the FP64
