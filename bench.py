@@ -480,14 +480,22 @@ def main():
             plain_cfg.write_text(base_cfg.read_text() + '\nrateInterval = 1000\n')
             mixed_cfg = Path(tmp) / "c1_mixed.cfg"
             mixed_cfg.write_text(base_cfg.read_text() + '\nrateInterval = 1000\nprecision = "mixed"\n')
-            for label, cfg in (("fp32", plain_cfg), ("mixed", mixed_cfg)):
+            # fp32_fused: the Reconstruction stage (H2D .. D2H as one stage,
+            # 3 lanes of 32-frame chain calls); its lanes hold up to 96 host
+            # frames, so the source replays 64 distinct frames per plane
+            legs = (("fp32", plain_cfg, ("--unique", "16")), ("mixed", mixed_cfg, ("--unique", "16")),
+                    ("fp32_fused", plain_cfg, ("--unique", "64", "--fused")))
+            for label, cfg, extra in legs:
                 try:
-                    r = subprocess.run([str(exe), str(cfg), "--frames", str(args.stream_frames), "--unique", "16"],
+                    r = subprocess.run([str(exe), str(cfg), "--frames", str(args.stream_frames), *extra],
                                        capture_output=True, text=True, timeout=300, env=env)
                     if r.returncode == 0:
                         res = json.loads(r.stdout.strip().splitlines()[-1])
                         res["what"] = ("C++ glados-style stages on device 0, frames streamed from pinned host "
-                                       "memory through H2D .. D2H, processorsPerDevice streams per stage")
+                                       "memory through H2D .. D2H, processorsPerDevice streams per stage"
+                                       if res.get("stages") != "fused" else
+                                       "the Reconstruction stage on device 0: H2D, the chain and D2H per batch "
+                                       "of queued frames, reconstructionLanes streams")
                         streamed[label] = res
                     else:
                         log(f"stage pipeline ({label}) failed: {r.stderr[-500:]}")

@@ -9,7 +9,11 @@
 // keeps the first few slices for checking.
 //
 //   risa_pipeline <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] [--tiff]
-//                 [--skip-every K] [--sync spin|yield|block]
+//                 [--skip-every K] [--sync spin|yield|block] [--fused]
+//
+// --fused: Source -> Reconstruction -> Sink, the device stages and both copies
+// as one stage that runs queued frames in batches through the chain (keys
+// reconstructionLanes, reconstructionBatch); not with --masking.
 //
 // precision = "mixed" in the configuration wires FilterMixed and
 // BackprojectionMixed (FP16 filtered sinograms) in place of Filter and
@@ -168,7 +172,7 @@ struct Options {
     long frames = 200;
     int unique = 8, keep = 0;
     long skip_every = 0;
-    bool masking = false, module_major = false, tiff = false;
+    bool masking = false, module_major = false, tiff = false, fused = false;
     unsigned sync_flag = 0;   // hipDeviceSchedule*, 0: the runtime's default
     std::string save_path;
 };
@@ -234,6 +238,31 @@ RunStats run_pipeline(const Options& o, Provenance& prov, SinkState& st)
     return r;
 }
 
+// Source -> Reconstruction (H2D .. D2H as one batched stage) -> Sink.
+RunStats run_fused(const Options& o, Provenance& prov, SinkState& st)
+{
+    using namespace glados::pipeline;
+    using Source = SourceStage<SyntheticLoader>;
+    using Sink = SinkStage<StatsSaver>;
+    Pipeline pipeline;
+    auto source = pipeline.create<Source>(o.config, o.frames, o.unique, &prov, o.module_major, o.skip_every);
+    auto rec = pipeline.create<Stage<risa::Reconstruction>>(o.config, o.module_major);
+    auto sink = pipeline.create<Sink>("", "IMG", &st, o.tiff ? o.config : std::string());
+    pipeline.connect(source, rec);
+    pipeline.connect(rec, sink);
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    pipeline.run(source, rec, sink);
+    pipeline.wait();
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    RunStats r;
+    r.secs = std::chrono::duration<double>(t1 - t0).count();
+    r.lost = rec->lostFrames();
+    r.loss_percent = rec->lossPercent();
+    r.d2h_rate = rec->reconstructionRate();
+    r.d2h_images_per_s = rec->imagesPerSecond();
+    return r;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -241,7 +270,7 @@ int main(int argc, char** argv)
     if (argc < 2) {
         std::fprintf(stderr,
                      "usage: %s <config> [--frames N] [--unique U] [--save K FILE] [--masking] [--module-major] "
-                     "[--tiff] [--skip-every K] [--sync spin|yield|block]\n",
+                     "[--tiff] [--skip-every K] [--sync spin|yield|block] [--fused]\n",
                      argv[0]);
         return 2;
     }
@@ -257,6 +286,7 @@ int main(int argc, char** argv)
         } else if (a == "--masking") o.masking = true;
         else if (a == "--module-major") o.module_major = true;
         else if (a == "--tiff") o.tiff = true;
+        else if (a == "--fused") o.fused = true;
         else if (a == "--skip-every" && i + 1 < argc) o.skip_every = std::atol(argv[++i]);
         else if (a == "--sync" && i + 1 < argc) {
             const std::string m = argv[++i];
@@ -271,6 +301,10 @@ int main(int argc, char** argv)
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
         }
+    }
+    if (o.fused && o.masking) {
+        std::fprintf(stderr, "--fused runs the fused mask only (no --masking)\n");
+        return 2;
     }
     if (o.skip_every == 1) {
         std::fprintf(stderr, "--skip-every needs K >= 2\n");
@@ -292,8 +326,9 @@ int main(int argc, char** argv)
     st.keep = o.keep;
     try {
         const bool mixed = risa::mixed_precision(o.config);
-        const RunStats r = mixed ? run_pipeline<risa::FilterMixed, risa::BackprojectionMixed>(o, prov, st)
-                                 : run_pipeline<risa::Filter, risa::Backprojection>(o, prov, st);
+        const RunStats r = o.fused ? run_fused(o, prov, st)
+                           : mixed ? run_pipeline<risa::FilterMixed, risa::BackprojectionMixed>(o, prov, st)
+                                   : run_pipeline<risa::Filter, risa::Backprojection>(o, prov, st);
         if (!o.save_path.empty()) {
             std::FILE* f = std::fopen(o.save_path.c_str(), "wb");
             if (!f) throw std::runtime_error("cannot open " + o.save_path);
@@ -307,10 +342,11 @@ int main(int argc, char** argv)
         }
         std::printf("{\"frames\": %ld, \"seconds\": %.4f, \"slices_per_s\": %.1f, \"latency_ms_mean\": %.3f, "
                     "\"latency_ms_max\": %.3f, \"saved\": %zu, \"precision\": \"%s\", \"lost_frames\": %lld, "
-                    "\"loss_percent\": %.4f, \"d2h_reconstruction_rate\": %.1f, \"d2h_images_per_s\": %.1f}\n",
+                    "\"loss_percent\": %.4f, \"d2h_reconstruction_rate\": %.1f, \"d2h_images_per_s\": %.1f, "
+                    "\"stages\": \"%s\"}\n",
                     st.count, r.secs, st.count / r.secs, st.count ? st.lat_sum / st.count : 0.0, st.lat_max,
                     st.kept.size(), mixed ? "mixed" : "fp32", r.lost, r.loss_percent, r.d2h_rate,
-                    r.d2h_images_per_s);
+                    r.d2h_images_per_s, o.fused ? "fused" : "staged");
         if (st.count != o.frames) {
             std::fprintf(stderr, "expected %ld slices, got %ld\n", o.frames, st.count);
             return 1;

@@ -522,6 +522,112 @@ void D2H::process(input_type&& img)
     DeviceStage::process(std::move(img));
 }
 
+// ------------------------------------------------------- Reconstruction --
+Reconstruction::Reconstruction(const std::string& configFile, bool module_major)
+    : rate_((std::size_t)std::max(1, config_int_or(configFile, "rateInterval", 10000)))
+{
+    g_ = read_geometry(configFile);
+    const int R = config_int(configFile, "numberOfReferenceFrames");
+    const std::vector<uint16_t> frames = reference_frames(configFile, g_, R);
+    const int devices = device_count();
+    const int lanes = std::max(1, config_int_or(configFile, "reconstructionLanes", 3));
+    batch_ = std::max(1, config_int_or(configFile, "reconstructionBatch", 32));
+    // every lane's batch and the slices on their way to the sink
+    const int pool = std::max(config_int(configFile, "memPoolSize_D2H"), 2 * lanes * batch_);
+    const int mode = mixed_precision(configFile) ? RISA_CHAIN_MIXED : RISA_CHAIN_PLAIN;
+    const std::size_t nf = fan_elems(g_), ni = img_elems(g_);
+    std::vector<float> ref((std::size_t)g_.numberOfPlanes * nf), dark((std::size_t)g_.numberOfPlanes * g_.numberOfFanDetectors);
+    route_by_index_ = true;
+    try {
+        for (int d = 0; d < devices; d++) {
+            glados::hip::ScopedDevice sd(d);
+            {
+                // the Attenuation stage's reference average, read back for the chain
+                SetupBuffer<uint16_t> dframes(frames.size());
+                SetupBuffer<float> dref(ref.size()), ddark(dark.size());
+                hip_check(hipMemcpy(dframes.p, frames.data(), frames.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+                          "hipMemcpy");
+                check(risa_hip_attenuation_prepare_reference(&g_, dframes.p, R, dref.p, ddark.p, nullptr),
+                      "Reconstruction::init");
+                hip_check(hipMemcpy(ref.data(), dref.p, ref.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+                hip_check(hipMemcpy(dark.data(), ddark.p, dark.size() * sizeof(float), hipMemcpyDeviceToHost),
+                          "hipMemcpy");
+            }
+            for (int l = 0; l < lanes; l++) {
+                risa_chain_t h = nullptr;
+                check(risa_hip_chain_create(&g_, ref.data(), dark.data(), d, mode, batch_, &h), "Reconstruction");
+                h_.push_back(h);
+                dev_.push_back(d);
+                raw_.push_back(nullptr);
+                img_.push_back(nullptr);
+                if (module_major)
+                    check(risa_hip_chain_set_input_layout(h, RISA_LAYOUT_MODULE_MAJOR), "Reconstruction");
+                hip_check(hipMalloc(&raw_.back(), (std::size_t)batch_ * nf * sizeof(uint16_t)), "hipMalloc");
+                hip_check(hipMalloc(&img_.back(), (std::size_t)batch_ * ni * sizeof(float)), "hipMalloc");
+            }
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    start_batched("Reconstruction", devices, lanes, pool, ni, batch_,
+                  [this, nf, ni](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      std::vector<int32_t> planes((std::size_t)n);
+                      for (int i = 0; i < n; i++) {
+                          if (ins[i].size() < nf) throw std::runtime_error("Reconstruction: frame smaller than the fan sinogram");
+                          hip_check(hipMemcpyAsync(raw_[(std::size_t)w] + (std::size_t)i * nf, ins[i].data(),
+                                                   nf * sizeof(uint16_t), hipMemcpyHostToDevice, s),
+                                    "hipMemcpyAsync");
+                          planes[(std::size_t)i] = (int32_t)ins[i].plane();
+                      }
+                      // eager launches: the plane pattern of a queued batch varies from call to call
+                      check(risa_hip_chain_run(h_[(std::size_t)w], raw_[(std::size_t)w], planes.data(), n,
+                                               img_[(std::size_t)w], 0, s),
+                            "Reconstruction::processor");
+                      for (int i = 0; i < n; i++)
+                          hip_check(hipMemcpyAsync(outs[i].data(), img_[(std::size_t)w] + (std::size_t)i * ni,
+                                                   ni * sizeof(float), hipMemcpyDeviceToHost, s),
+                                    "hipMemcpyAsync");
+                  });
+}
+
+void Reconstruction::release()
+{
+    for (std::size_t i = 0; i < h_.size(); i++) {
+        glados::hip::ScopedDevice sd(dev_[i]);
+        risa_hip_chain_destroy(h_[i]);
+        if (raw_[i]) (void)hipFree(raw_[i]);
+        if (img_[i]) (void)hipFree(img_[i]);
+    }
+    h_.clear();
+    raw_.clear();
+    img_.clear();
+    dev_.clear();
+}
+
+Reconstruction::~Reconstruction()
+{
+    stop();
+    release();
+    std::fprintf(stderr, "Reconstruction: Could not reconstruct %lld elements; %g%% loss\n", loss_.lost(),
+                 loss_.loss_percent());
+    std::fprintf(stderr, "Reconstruction: Reconstructed %g Images/s in average (%zu images, %g images/s over the run).\n",
+                 rate_.reconstruction_rate(), rate_.images(), rate_.images_per_second());
+}
+
+void Reconstruction::process(input_type&& img)
+{
+    if (img.valid()) loss_.arrived(img.index());
+    DeviceStage::process(std::move(img));
+}
+
+Reconstruction::output_type Reconstruction::wait()
+{
+    output_type out = DeviceStage::wait();
+    if (out.valid()) rate_.arrived(out.index());
+    return out;
+}
+
 // ---------------------------------------------------------- OfflineSaver --
 static std::string config_string(const std::string& path, const char* key)
 {

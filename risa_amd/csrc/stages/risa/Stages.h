@@ -12,6 +12,7 @@
 //   Backprojection Backprojection.cu:46-208      parallel sinogram -> slice
 //   Masking        Masking.cu:46-169             in place
 //   D2H            Copy/D2H.cu:83-148            device slice -> host
+//   Reconstruction (no counterpart)              H2D .. D2H as one batched stage
 //   OfflineSaver   OfflineSaver.cu:66-178        sink: TIFF sequences / latest image
 #pragma once
 
@@ -183,6 +184,43 @@ public:
     std::size_t images() const { return rate_.images(); }
 
 private:
+    RateMeter rate_;
+};
+
+// The device part of the chain -- H2D, Attenuation, Fan2Para, Filter,
+// Backprojection (mask fused), D2H -- as one stage, host frames in, host
+// slices out.  No reference counterpart: it spans the six stages
+// RISA/example/main.cpp:95-134 wires between loader and saver and replaces
+// their per-frame queue hand-offs and stream synchronisations by one per
+// batch.  A processor takes up to reconstructionBatch queued frames (default
+// 32), copies them to its device staging, runs them as one chain call
+// (risa_hip_chain_run: plain FP32, or mixed with precision = "mixed";
+// module-major input when constructed so) and copies the slices out, all on
+// its stream; reconstructionLanes processors per device (default 3) keep
+// batches in flight, so one lane's copies overlap another's kernels.  A
+// frame's host buffer is held until its batch is done: the source's pool
+// must exceed lanes x batch frames per device.  H2D's lost-frame accounting
+// (arrivals) and D2H's reconstruction rate (slices handed on) are kept.
+class Reconstruction : public DeviceStage<HostImage<uint16_t>, HostImage<float>> {
+public:
+    explicit Reconstruction(const std::string& configFile, bool module_major = false);
+    ~Reconstruction();
+    void process(input_type&& img);
+    output_type wait();
+    long long lostFrames() const { return loss_.lost(); }
+    double lossPercent() const { return loss_.loss_percent(); }
+    double reconstructionRate() const { return rate_.reconstruction_rate(); }
+    double imagesPerSecond() const { return rate_.images_per_second(); }
+
+private:
+    void release();
+    risa_geom g_{};
+    int batch_ = 32;
+    std::vector<risa_chain_t> h_;   // per processor
+    std::vector<uint16_t*> raw_;    // per processor: [batch][P_f][D_f] device staging
+    std::vector<float*> img_;       // per processor: [batch][N][N] device staging
+    std::vector<int> dev_;          // per processor: its device
+    FrameLossCounter loss_;
     RateMeter rate_;
 };
 

@@ -587,11 +587,13 @@ def test_c3_filtered_uneven_geometries(risa, orc, geom_c0, cuda):
 
 def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
     """The run calls that use scratch buffers (GEMM-mode in-place filter, tap
-    gradient, FP16 plain-layout backprojection, the gather adjoint (D > N) and
-    the sheared-path adjoint in two 8-slice passes (D <= N)) record into a
-    hipGraph: they allocate nothing and do not synchronise.  Each replay --
-    with eager work and synchronisations between replays -- equals the eager
-    result bit for bit."""
+    gradient, FP16 plain-layout backprojection, the gather adjoint (D > N),
+    the sheared-path adjoint in two 8-slice passes (D <= N), the long-row
+    filter's global-memory passes) and the persistent uneven-batch launch
+    (its tile counter reset by a kernel) record into a hipGraph: they
+    allocate nothing and do not synchronise.  Each replay -- with eager work
+    and synchronisations between replays -- equals the eager result bit for
+    bit."""
     import torch
     from risa_amd import device
 
@@ -611,6 +613,17 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
     bpp = device.Backprojection(gp)
     SA = 10
     imgs = _t(rng.standard_normal((SA, N, N)).astype(np.float32), cuda)
+    gl = g.replace(numberOfParallelDetectors=4099, numberOfParallelProjections=7)   # transform length 8640
+    fl = device.Filter(gl, mode=0)
+    xl0 = _t(rng.standard_normal((3,) + gl.par_shape).astype(np.float32), cuda)
+    jobs = []
+    for i, (Nj, Pj, Dj) in enumerate([(64, 90, 70), (96, 120, 130), (128, 60, 97), (80, 150, 101)]):
+        f16 = i % 2
+        sino = rng.standard_normal((Pj, Dj)).astype(np.float32)
+        jobs.append(dict(sino=_t(sino.astype(np.float16) if f16 else sino, cuda),
+                         img=torch.empty((Nj, Nj), dtype=torch.float32, device=cuda), N=Nj, P=Pj, D=Dj,
+                         dtype=f16, in_scale=1.0, rotationOffset=7.0 * i, mask=i % 2, maskingValue=-1.0))
+    bb = device.BatchedBackprojection(jobs)
 
     def work(o):
         o["x"].copy_(x0)
@@ -619,13 +632,22 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
         bp.f16(h16, 256.0, S16, o["img16"])     # plain-layout FP16 through the staging buffer
         bp.adjoint(img0, o["adj"])              # transposed image buffer
         bpp.adjoint_batch(imgs, SA, o["adjp"])  # padded copies, tile origins, partials
+        o["xl"].copy_(xl0)
+        fl(o["xl"], 3)                          # global-memory transform passes, in place
+        bb()                                    # persistent launch over the uneven jobs
+        for i, j in enumerate(jobs):
+            o[f"bb{i}"].copy_(j["img"])
 
     def outs():
-        return dict(x=torch.empty_like(x0), dt=torch.empty(D // 2 + 1, device=cuda),
-                    img16=torch.empty((S16, N, N), device=cuda), adj=torch.empty((P, D), device=cuda),
-                    adjp=torch.empty((SA, P, 160), device=cuda))
+        o = dict(x=torch.empty_like(x0), dt=torch.empty(D // 2 + 1, device=cuda),
+                 img16=torch.empty((S16, N, N), device=cuda), adj=torch.empty((P, D), device=cuda),
+                 adjp=torch.empty((SA, P, 160), device=cuda), xl=torch.empty_like(xl0))
+        for i, j in enumerate(jobs):
+            o[f"bb{i}"] = torch.empty_like(j["img"])
+        return o
 
-    print("case: run calls under graph capture (GEMM filter, grad taps, FP16 BP, adjoint)", flush=True)
+    print("case: run calls under graph capture (GEMM filter, grad taps, FP16 BP, adjoints, long-row filter, "
+          "persistent batch)", flush=True)
     eager = outs()
     work(eager)
     _sync()
@@ -646,7 +668,7 @@ def test_run_calls_capture_in_graph(risa, orc, geom_c0, cuda):
         work(outs())                            # eager work between replays
         _sync()
     del graph
-    for h in (fg, ff, bp, bpp):
+    for h in (fg, ff, bp, bpp, fl, bb):
         h.close()
 
 

@@ -1,8 +1,9 @@
 """The C++ glados-style stage pipeline on the GPU (RISA/example/main.cpp's
 wiring): frames streamed host -> device -> host through H2D, optionally
 Reordering (module-major ROFEX frames), Attenuation, Fan2Para, Filter,
-Backprojection (mask fused), optionally Masking, and D2H.  The slices it
-saves are checked against the FP64 oracle chain."""
+Backprojection (mask fused), optionally Masking, and D2H -- or, with
+--fused, the Reconstruction stage that runs the same span in batches through
+the chain.  The slices it saves are checked against the FP64 oracle chain."""
 from __future__ import annotations
 
 import json
@@ -41,7 +42,9 @@ def read_saved(path, N):
 
 @pytest.mark.parametrize("cfg,extra,precision", [("c0", (), "fp32"), ("c1", ("--masking",), "fp32"),
                                                  ("c0", ("--module-major",), "fp32"), ("c0", (), "mixed"),
-                                                 ("c1", (), "mixed")])
+                                                 ("c1", (), "mixed"), ("c1", ("--fused",), "fp32"),
+                                                 ("c0", ("--fused", "--module-major"), "fp32"),
+                                                 ("c1", ("--fused",), "mixed")])
 def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extra, precision):
     """precision = "mixed" wires FilterMixed (DFT-GEMM on MFMA, FP16 sinogram)
     and BackprojectionMixed (FP16 sinogram, FP32 accumulation): within the
@@ -59,6 +62,7 @@ def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extr
     res = run_pipeline(path, frames, 6, out, extra)
     assert res["frames"] == frames
     assert res["precision"] == precision
+    assert res["stages"] == ("fused" if "--fused" in extra else "staged")
     assert res["lost_frames"] == 0
     tol = 1e-5 if precision == "fp32" else 2e-4
     saved = read_saved(out, N)
@@ -78,13 +82,15 @@ def test_stage_pipeline_matches_fp64_oracle(risa, orc, cuda, tmp_path, cfg, extr
     oc.close()
 
 
-def test_stage_pipeline_counts_lost_frames(risa, cuda, tmp_path):
+@pytest.mark.parametrize("extra", [(), ("--fused",)])
+def test_stage_pipeline_counts_lost_frames(risa, cuda, tmp_path, extra):
     """A source that leaves out every 5th index (frames lost before the GPU):
-    H2D counts the missing indices and the loss against the last index
-    (H2D.cu:102-107), D2H reports its rate (D2H.cu:85-93); every emitted
-    frame is reconstructed."""
+    H2D (or the fused Reconstruction stage) counts the missing indices and
+    the loss against the last index (H2D.cu:102-107), D2H (or
+    Reconstruction) reports its rate (D2H.cu:85-93); every emitted frame is
+    reconstructed."""
     frames, k = 40, 5
-    res = run_pipeline(risa.config_path("c0"), frames, 0, tmp_path / "unused.bin", ("--skip-every", str(k)))
+    res = run_pipeline(risa.config_path("c0"), frames, 0, tmp_path / "unused.bin", ("--skip-every", str(k), *extra))
     # emitted indices: 0, 1, 2, 3, 5, ... (i % 5 == 4 left out)
     idx, i = [], 0
     while len(idx) < frames:
