@@ -203,7 +203,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=2000, help="frames per plane in one step")
     ap.add_argument("--unique", type=int, default=16, help="distinct seeded frames per plane")
-    ap.add_argument("--chunk", type=int, default=256, help="frames per chain call (graph)")
+    ap.add_argument("--chunk", type=int, default=512, help="frames per chain call (graph)")
     ap.add_argument("--cpu-frames", type=int, default=256,
                     help="frame-planes in the CPU baseline sample (about 10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true")

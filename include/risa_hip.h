@@ -403,10 +403,11 @@ int risa_hip_bp_batched_destroy(risa_bp_batched_t h);
 
 /* ------------------------------------------------------------------ Chain -- */
 /* The whole per-frame chain on one device with its own scratch buffers.
- * mode 0 = plain FP32 frame by frame (FFT filter, FP32 backprojection);
- * mode 1 = mixed precision, batched (DFT-GEMM filter with FP16 output,
- * FP16-sinogram / FP32-accumulate backprojection).  max_frames bounds one
- * run call. */
+ * mode 0 = plain FP32 (FFT filter, FP32 backprojection), groups of up to 16
+ * frames; mode 1 = mixed precision (DFT-GEMM filter with FP16 output,
+ * FP16-sinogram / FP32-accumulate backprojection), groups of up to 32
+ * frames.  A run's groups are dealt to two streams forked from and joined
+ * into the caller's stream.  max_frames bounds one run call. */
 enum risa_chain_mode { RISA_CHAIN_PLAIN = 0, RISA_CHAIN_MIXED = 1 };
 
 int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float* avg_dark,
@@ -417,8 +418,9 @@ int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, const float*
 int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames,
                        float* img, int use_graph, void* stream);
 /* Intermediate buffers of the last run: which = 0 attenuation, 1 rebinned,
- * 2 filtered (PLAIN: FP32, sixteen frame slots per lane, frame 0 of a run in
- * slot 0; MIXED: FP16 4-slice layout, one slot per frame). */
+ * 2 filtered (PLAIN: FP32; MIXED: FP16 4-slice layout).  Each holds one
+ * group of frames per stream: frame f of a run sits in slot f while f is
+ * below the chain's frames in flight (later groups reuse the slots). */
 int risa_hip_chain_intermediate(risa_chain_t h, int which, const void** ptr);
 /* Backprojector variant of the chain (as risa_hip_bp_set_variant; replaces
  * the autotuner's pick).  A change synchronises the chain's device and

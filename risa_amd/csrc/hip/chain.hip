@@ -11,9 +11,12 @@
 //        buffers each) forked from and joined back into the caller's stream,
 //        so lanes x per frames are in flight -- the way the reference
 //        overlaps frames across its per-stage threads.
-// MIXED: all frames of a call as one batch; DFT-GEMM filter on MFMA writing
-//        the FP16 sinogram in the 4-slice interleaved layout, FP16-sinogram /
-//        FP32-accumulate backprojection.
+// MIXED: groups of `per` consecutive frames (a multiple of 4, up to 32) dealt
+//        to `lanes` streams the same way; per group the DFT-GEMM filter on
+//        MFMA writes the FP16 sinogram in the 4-slice interleaved layout and
+//        the FP16-sinogram / FP32-accumulate backprojection runs the group in
+//        one launch, so one lane's front stages overlap the other's
+//        backprojection.
 #include "device_common.h"
 
 #include <map>
@@ -26,6 +29,8 @@ namespace {
 constexpr float kF16SinoScale = 256.0f;   // FP16 filtered sinogram stores 256 * value
 constexpr int kPlainLanes = 2;            // streams in PLAIN mode
 constexpr int kPlainMaxPer = 16;          // frames per lane step in PLAIN mode, at most
+constexpr int kMixedLanes = 2;            // streams in MIXED mode
+constexpr int kMixedMaxPer = 32;          // frames per lane step in MIXED mode, at most (a multiple of 4)
 }  // namespace
 
 // Graph executables a chain keeps (one per buffer set and call shape); the
@@ -42,14 +47,14 @@ struct risa_chain_s {
     int mode = 0;
     int max_frames = 0;
     int lanes = 1;
-    int per = 1;   // PLAIN: frames per lane step
+    int per = 1;   // frames per lane step
     risa_geom g{};
     risa_attenuation_t att = nullptr;
     risa_fan2para_t f2p = nullptr;
     risa_filter_t filt = nullptr;
     risa_bp_t bp = nullptr;
-    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;   // PLAIN: kPlainMaxPer slots per lane
-    risa::DeviceBuffer<uint16_t> f16_buf;
+    risa::DeviceBuffer<float> att_buf, par_buf, filt_buf;   // `per` frame slots per lane
+    risa::DeviceBuffer<uint16_t> f16_buf;                   // MIXED: `per` slots per lane, x4 layout
     std::vector<hipStream_t> lane_streams;
     std::vector<hipEvent_t> lane_events;   // [0] fork, [1..lanes] joins
     std::map<std::tuple<const void*, const void*, int, std::vector<int32_t>>, ChainGraph> graphs;
@@ -114,35 +119,41 @@ extern "C" int risa_hip_chain_create(const risa_geom* g, const float* avg_ref, c
             const size_t par = (size_t)g->numberOfParallelDetectors * g->numberOfParallelProjections;
             if (mode == RISA_CHAIN_PLAIN) {
                 h->lanes = std::min(kPlainLanes, max_frames);
-                const size_t slots = (size_t)h->lanes * kPlainMaxPer;
+                h->per = std::max(1, std::min(kPlainMaxPer, max_frames / h->lanes));
+                const size_t slots = (size_t)h->lanes * h->per;
                 h->att_buf.resize(fan * slots);
                 h->par_buf.resize(par * slots);
                 h->filt_buf.resize(par * slots);
-                for (int k = 0; k < h->lanes; k++) {
-                    hipStream_t s;
-                    RISA_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-                    h->lane_streams.push_back(s);
-                }
-                for (int k = 0; k <= h->lanes; k++) {
-                    hipEvent_t e;
-                    RISA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-                    h->lane_events.push_back(e);
-                }
             } else {
-                h->att_buf.resize(fan * max_frames);
-                h->par_buf.resize(par * max_frames);
-                h->f16_buf.resize(par * 4 * (size_t)((max_frames + 3) / 4));   // x4 layout
+                // groups of a multiple of 4 frames: whole x4 slice groups per lane
+                const int half = (max_frames + kMixedLanes - 1) / kMixedLanes;
+                h->per = std::min(kMixedMaxPer, (half + 3) / 4 * 4);
+                h->lanes = std::min(kMixedLanes, (max_frames + h->per - 1) / h->per);
+                const size_t slots = (size_t)h->lanes * h->per;
+                h->att_buf.resize(fan * slots);
+                h->par_buf.resize(par * slots);
+                h->f16_buf.resize(par * slots);   // x4 layout: per / 4 groups of 4 slices per lane
+            }
+            for (int k = 0; k < h->lanes; k++) {
+                hipStream_t s;
+                RISA_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                h->lane_streams.push_back(s);
+            }
+            for (int k = 0; k <= h->lanes; k++) {
+                hipEvent_t e;
+                RISA_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                h->lane_events.push_back(e);
             }
             // backprojector tile shape, timed at the frames the chain keeps
             // in flight (plain: lanes x per slices, two-slice tiles included
-            // when per > 1) or at its batch (mixed); timing the chain itself
-            // per variant, eagerly, proved too noisy to rank the closest shapes
+            // when per > 1) or at a lane step's batch (mixed); timing the chain
+            // itself per variant, eagerly, proved too noisy to rank the closest
+            // shapes
             if (mode == RISA_CHAIN_PLAIN) {
-                h->per = std::max(1, std::min(kPlainMaxPer, max_frames / h->lanes));
                 if (h->per > 1) ok(risa_hip_bp_autotune_batch(h->bp, h->lanes * h->per, nullptr));
                 else ok(risa_hip_bp_autotune_slices(h->bp, h->lanes, nullptr));
             } else {
-                ok(risa_hip_bp_autotune_f16(h->bp, std::min(max_frames, 64), nullptr));
+                ok(risa_hip_bp_autotune_f16(h->bp, h->per, nullptr));
             }
             RISA_HIP_CHECK(hipDeviceSynchronize());
         } catch (...) {
@@ -159,18 +170,18 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
     const size_t fan = (size_t)g.numberOfFanDetectors * g.numberOfFanProjections;
     const size_t par = (size_t)g.numberOfParallelDetectors * g.numberOfParallelProjections;
     const size_t out = (size_t)g.numberOfPixels * g.numberOfPixels;
-    if (h->mode == RISA_CHAIN_PLAIN) {
-        const int per = h->per;
-        const int groups = (frames + per - 1) / per;
-        const int lanes = std::min(h->lanes, groups);
-        RISA_HIP_CHECK(hipEventRecord(h->lane_events[0], s));
-        for (int k = 0; k < lanes; k++) RISA_HIP_CHECK(hipStreamWaitEvent(h->lane_streams[k], h->lane_events[0], 0));
-        for (int q = 0; q < groups; q++) {
-            const int k = q % lanes, f = q * per, n = std::min(per, frames - f);
-            hipStream_t ls = h->lane_streams[k];
-            float* att = h->att_buf.ptr + (size_t)k * kPlainMaxPer * fan;
-            float* pb = h->par_buf.ptr + (size_t)k * kPlainMaxPer * par;
-            float* fb = h->filt_buf.ptr + (size_t)k * kPlainMaxPer * par;
+    const int per = h->per;
+    const int groups = (frames + per - 1) / per;
+    const int lanes = std::min(h->lanes, groups);
+    RISA_HIP_CHECK(hipEventRecord(h->lane_events[0], s));
+    for (int k = 0; k < lanes; k++) RISA_HIP_CHECK(hipStreamWaitEvent(h->lane_streams[k], h->lane_events[0], 0));
+    for (int q = 0; q < groups; q++) {
+        const int k = q % lanes, f = q * per, n = std::min(per, frames - f);
+        hipStream_t ls = h->lane_streams[k];
+        float* att = h->att_buf.ptr + (size_t)k * per * fan;
+        float* pb = h->par_buf.ptr + (size_t)k * per * par;
+        if (h->mode == RISA_CHAIN_PLAIN) {
+            float* fb = h->filt_buf.ptr + (size_t)k * per * par;
             if (n == 1) {
                 ok(risa_hip_attenuation_run(h->att, raw + f * fan, planes[f], att, ls));
                 ok(risa_hip_fan2para_run(h->f2p, att, planes[f], pb, ls));
@@ -182,17 +193,18 @@ static void enqueue(risa_chain_t h, const uint16_t* raw, const int32_t* planes, 
                 ok(risa_filter_run_oop(h->filt, pb, fb, (long)n * g.numberOfParallelProjections, ls));
                 ok(risa_hip_bp_run_batch(h->bp, fb, n, img + f * out, ls));
             }
+        } else {
+            uint16_t* hb = h->f16_buf.ptr + (size_t)k * per * par;
+            ok(risa_hip_attenuation_run_batch(h->att, raw + f * fan, planes + f, n, att, ls));
+            ok(risa_hip_fan2para_run_batch(h->f2p, att, planes + f, n, pb, ls));
+            ok(risa_hip_filter_run_f16x4(h->filt, pb, hb, n, kF16SinoScale, ls));
+            ok(risa_hip_bp_run_f16x4(h->bp, hb, kF16SinoScale, n, img + f * out, ls));
         }
-        for (int k = 0; k < lanes; k++) {
-            RISA_HIP_CHECK(hipEventRecord(h->lane_events[k + 1], h->lane_streams[k]));
-            RISA_HIP_CHECK(hipStreamWaitEvent(s, h->lane_events[k + 1], 0));
-        }
-        return;
     }
-    ok(risa_hip_attenuation_run_batch(h->att, raw, planes, frames, h->att_buf.ptr, s));
-    ok(risa_hip_fan2para_run_batch(h->f2p, h->att_buf.ptr, planes, frames, h->par_buf.ptr, s));
-    ok(risa_hip_filter_run_f16x4(h->filt, h->par_buf.ptr, h->f16_buf.ptr, frames, kF16SinoScale, s));
-    ok(risa_hip_bp_run_f16x4(h->bp, h->f16_buf.ptr, kF16SinoScale, frames, img, s));
+    for (int k = 0; k < lanes; k++) {
+        RISA_HIP_CHECK(hipEventRecord(h->lane_events[k + 1], h->lane_streams[k]));
+        RISA_HIP_CHECK(hipStreamWaitEvent(s, h->lane_events[k + 1], 0));
+    }
 }
 
 extern "C" int risa_hip_chain_run(risa_chain_t h, const uint16_t* raw, const int32_t* planes, int frames, float* img,

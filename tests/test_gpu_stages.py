@@ -425,6 +425,50 @@ def test_chain_plain_lane_groups_equal_single_frames(risa, geom_c0, frames_c0, c
     b.close()
 
 
+def test_chain_mixed_lane_groups_equal_library_calls(risa, geom_c0, frames_c0, cuda):
+    """A mixed chain call of 70 frames (2 lanes, groups of 36 and 34 frames,
+    the last x4 slice group half empty) gives every frame bit for bit what
+    the four library calls give it over the whole batch (attenuation,
+    rebinning, DFT-GEMM filter with x4 FP16 output, FP16 backprojection with
+    the chain's tile variant), eager and as a replayed graph."""
+    import torch
+    from risa_amd import device
+
+    g, fs = geom_c0, frames_c0
+    F = 70
+    idx = np.arange(F) % len(fs.planes)
+    raw = _t(fs.raw[idx], cuda)
+    planes = fs.planes[idx].astype(np.int32)
+    a = device.Chain(g, fs.avg_ref, fs.avg_dark, mode=risa.CHAIN_MIXED, max_frames=F)
+    print(f"case: chain mixed C0, {F} frames in lane groups, variant {a.variant >> 8}", flush=True)
+    img = torch.full((F,) + g.img_shape, float("nan"), dtype=torch.float32, device=cuda)
+    a(raw, planes, img)
+    att = device.Attenuation(g, fs.avg_ref, fs.avg_dark)
+    f2p = device.Fan2Para(g)
+    flt = device.Filter(g, mode=1)
+    bp = device.Backprojection(g)
+    bp.variant = a.variant
+    fan = torch.empty((F,) + g.fan_shape, dtype=torch.float32, device=cuda)
+    att.batch(raw, planes, fan)
+    par = torch.empty((F,) + g.par_shape, dtype=torch.float32, device=cuda)
+    f2p.batch(fan, planes, par)
+    x4 = torch.zeros(((F + 3) // 4 * 4,) + g.par_shape, dtype=torch.float16, device=cuda)
+    flt.f16x4(par, x4, F, 256.0)
+    ref = torch.full_like(img, float("nan"))
+    bp.f16x4(x4, 256.0, F, ref)
+    _sync()
+    for f in range(F):
+        assert torch.equal(img[f], ref[f]), f"frame {f}"
+    img2 = torch.full_like(img, float("nan"))
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a(raw, planes, img2, use_graph=True)
+    s.synchronize()
+    assert torch.equal(img2, img)
+    for h in (a, att, f2p, flt, bp):
+        h.close()
+
+
 def test_chain_plain_matches_fp64_oracle(risa, orc, geom_c0, frames_c0, cuda):
     import torch
     from risa_amd import device
