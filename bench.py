@@ -138,9 +138,10 @@ def kernel_only(args, g, dev):
                 layout = "module-major (reordering fused)" if args.kernel == "att_mm" else "detector-major"
                 what = f"attenuation, {F} frames of {Pf} x {Df}, {layout} input"
         elif args.kernel in ("rebin", "dft"):
-            # the mixed chain's front stages at its 64-frame shape (C1, both
-            # planes): rebinning of attenuated FP32 frames, and the DFT-GEMM
-            # filter to the FP16 x4 layout
+            # the mixed chain's front stages alone, 64 frames per launch (C1,
+            # both planes; the chain's lane steps run 32): rebinning of
+            # attenuated FP32 frames, and the DFT-GEMM filter to the FP16 x4
+            # layout
             from risa_amd import synth
             F, Pf, Df = 64, g.numberOfFanProjections, g.numberOfFanDetectors
             P, D = g.numberOfParallelProjections, g.numberOfParallelDetectors
