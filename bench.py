@@ -485,7 +485,8 @@ def main():
             # 3 lanes of 32-frame chain calls); its lanes hold up to 96 host
             # frames, so the source replays 64 distinct frames per plane
             legs = (("fp32", plain_cfg, ("--unique", "16")), ("mixed", mixed_cfg, ("--unique", "16")),
-                    ("fp32_fused", plain_cfg, ("--unique", "64", "--fused")))
+                    ("fp32_fused", plain_cfg, ("--unique", "64", "--fused")),
+                    ("mixed_fused", mixed_cfg, ("--unique", "64", "--fused")))
             for label, cfg, extra in legs:
                 try:
                     r = subprocess.run([str(exe), str(cfg), "--frames", str(args.stream_frames), *extra],

@@ -12,6 +12,7 @@
 // pool is released instead of re-pooled.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstddef>
@@ -39,7 +40,9 @@ public:
     }
 
     // Allocates `count` buffers of `elements` values on `device` for one
-    // stage; returns the stage's pool id.
+    // stage (in one allocation, in address order, where the manager can);
+    // returns the stage's pool id.  The first request gets the lowest
+    // address.
     int registerStage(size_type count, size_type elements, int device = 0)
     {
         Slot s;
@@ -47,7 +50,8 @@ public:
         s.device = device;
         Manager mm;
         mm.setDevice(device);
-        for (size_type i = 0; i < count; i++) s.free.push_back(mm.make_ptr(elements));
+        s.free = make_buffers(mm, count, elements, 0);
+        std::reverse(s.free.begin(), s.free.end());   // taken from the back
         std::lock_guard<std::mutex> lock(m_);
         slots_.push_back(std::move(s));
         return (int)slots_.size() - 1;
@@ -99,6 +103,18 @@ public:
     }
 
 private:
+    template <class M>
+    static auto make_buffers(const M& mm, size_type count, size_type n, int) -> decltype(mm.make_ptrs(count, n))
+    {
+        return mm.make_ptrs(count, n);
+    }
+    template <class M>
+    static std::vector<pointer_type> make_buffers(const M& mm, size_type count, size_type n, long)
+    {
+        std::vector<pointer_type> v;
+        for (size_type i = 0; i < count; i++) v.push_back(mm.make_ptr(n));
+        return v;
+    }
     struct Slot {
         std::vector<pointer_type> free;
         size_type elements = 0;

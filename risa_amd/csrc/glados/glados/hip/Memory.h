@@ -12,6 +12,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace glados {
 namespace hip {
@@ -69,8 +70,14 @@ class HostMemoryManager {
 public:
     using value_type = T;
     using size_type = std::size_t;
+    // A buffer is its own allocation, or (slab set) a piece of a shared one
+    // that is freed with its last piece.
     struct Deleter {
-        void operator()(T* p) const noexcept { (void)hipHostFree(p); }
+        std::shared_ptr<void> slab;
+        void operator()(T* p) const noexcept
+        {
+            if (!slab) (void)hipHostFree(p);
+        }
     };
     using pointer_type_1D = std::unique_ptr<T[], Deleter>;
 
@@ -82,6 +89,19 @@ public:
         void* p = nullptr;
         if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
         return pointer_type_1D(static_cast<T*>(p));
+    }
+    // `count` buffers of n values carved in address order from one pinned
+    // allocation (a pool's buffers): a copy of adjacent buffers can then go
+    // as one transfer.
+    std::vector<pointer_type_1D> make_ptrs(size_type count, size_type n) const
+    {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, count * n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        const std::shared_ptr<void> slab(p, [](void* q) { (void)hipHostFree(q); });
+        std::vector<pointer_type_1D> v;
+        v.reserve(count);
+        for (size_type i = 0; i < count; i++) v.emplace_back(static_cast<T*>(p) + i * n, Deleter{slab});
+        return v;
     }
     void copy(pointer_type_1D& dst, const pointer_type_1D& src, size_type n) const
     {

@@ -572,22 +572,34 @@ Reconstruction::Reconstruction(const std::string& configFile, bool module_major)
     }
     start_batched("Reconstruction", devices, lanes, pool, ni, batch_,
                   [this, nf, ni](int w, int, input_type* ins, output_type* outs, int n, hipStream_t s) {
+                      // Frames take staging slots, and slices output buffers, in
+                      // host address order (the pair ins[i] -> outs[i] is all the
+                      // caller keeps), so each run of adjacent pooled buffers
+                      // crosses PCIe as one copy.
+                      auto by_address = [](const auto& a, const auto& b) { return a.data() < b.data(); };
+                      std::sort(ins, ins + n, by_address);
+                      std::sort(outs, outs + n, by_address);
                       std::vector<int32_t> planes((std::size_t)n);
                       for (int i = 0; i < n; i++) {
                           if (ins[i].size() < nf) throw std::runtime_error("Reconstruction: frame smaller than the fan sinogram");
-                          hip_check(hipMemcpyAsync(raw_[(std::size_t)w] + (std::size_t)i * nf, ins[i].data(),
-                                                   nf * sizeof(uint16_t), hipMemcpyHostToDevice, s),
-                                    "hipMemcpyAsync");
                           planes[(std::size_t)i] = (int32_t)ins[i].plane();
+                      }
+                      for (int i = 0, j; i < n; i = j) {
+                          for (j = i + 1; j < n && ins[j].data() == ins[j - 1].data() + nf; j++) {}
+                          hip_check(hipMemcpyAsync(raw_[(std::size_t)w] + (std::size_t)i * nf, ins[i].data(),
+                                                   (std::size_t)(j - i) * nf * sizeof(uint16_t), hipMemcpyHostToDevice, s),
+                                    "hipMemcpyAsync");
                       }
                       // eager launches: the plane pattern of a queued batch varies from call to call
                       check(risa_hip_chain_run(h_[(std::size_t)w], raw_[(std::size_t)w], planes.data(), n,
                                                img_[(std::size_t)w], 0, s),
                             "Reconstruction::processor");
-                      for (int i = 0; i < n; i++)
+                      for (int i = 0, j; i < n; i = j) {
+                          for (j = i + 1; j < n && outs[j].data() == outs[j - 1].data() + ni; j++) {}
                           hip_check(hipMemcpyAsync(outs[i].data(), img_[(std::size_t)w] + (std::size_t)i * ni,
-                                                   ni * sizeof(float), hipMemcpyDeviceToHost, s),
+                                                   (std::size_t)(j - i) * ni * sizeof(float), hipMemcpyDeviceToHost, s),
                                     "hipMemcpyAsync");
+                      }
                   });
 }
 
