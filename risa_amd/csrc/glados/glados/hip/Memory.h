@@ -65,6 +65,16 @@ protected:
     int device_ = 0;
 };
 
+// Elements between the starts of two pieces of a pool allocation: n values
+// rounded up to 256 bytes, so every piece keeps the alignment copies and
+// kernels expect.  Pieces are adjacent (stride n) when n values fill whole
+// 256-byte blocks.
+template <class T>
+inline std::size_t piece_stride(std::size_t n)
+{
+    return (n * sizeof(T) + 255) / 256 * 256 / sizeof(T);
+}
+
 template <class T>
 class HostMemoryManager {
 public:
@@ -95,12 +105,13 @@ public:
     // as one transfer.
     std::vector<pointer_type_1D> make_ptrs(size_type count, size_type n) const
     {
+        const size_type stride = piece_stride<T>(n);
         void* p = nullptr;
-        if (hipHostMalloc(&p, count * n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        if (hipHostMalloc(&p, count * stride * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
         const std::shared_ptr<void> slab(p, [](void* q) { (void)hipHostFree(q); });
         std::vector<pointer_type_1D> v;
         v.reserve(count);
-        for (size_type i = 0; i < count; i++) v.emplace_back(static_cast<T*>(p) + i * n, Deleter{slab});
+        for (size_type i = 0; i < count; i++) v.emplace_back(static_cast<T*>(p) + i * stride, Deleter{slab});
         return v;
     }
     void copy(pointer_type_1D& dst, const pointer_type_1D& src, size_type n) const
