@@ -2,7 +2,7 @@
 // (RISA/risaLib/src/DetectorInterpolation/interpolationFunctions.h:28-97),
 // compiled against that header where it lies in the reference checkout
 // (see oracle/Makefile, target `ref`).  Used only to write the golden
-// vectors under tests/golden/; never shipped, never run on the GPU box.
+// vectors under tests/golden/; never shipped, never part of a GPU run.
 //
 // Input file : uint32 D, uint32 P, double threshMin, double threshMax,
 //              uint16 data[P][D]
