@@ -52,10 +52,11 @@ def test_filter_fft32_any_length(risa, orc, geom_c0, cuda, D):
 
 
 # Transforms longer than 4096 run pass by pass through global memory:
-# 2049 = 3 * 683 (padded to 4320), 4099 (prime, padded to 8640), 5000 and
+# 2049 = 3 * 683 (padded to 4320), 4099 (prime, padded to 8640), 10007
+# (prime, padded to 20250), 12345 = 3 * 5 * 823 (padded to 25000), 5000 and
 # 16384 (direct); 3000 and 4096 stay in LDS.  Three slices of 37 rows leave
 # the last row pair half empty and pairs straddling sinograms.
-LONG_D = [(2049, 37), (3000, 37), (4096, 37), (4099, 37), (5000, 37), (16384, 5)]
+LONG_D = [(2049, 37), (3000, 37), (4096, 37), (4099, 37), (5000, 37), (16384, 5), (10007, 5), (12345, 3)]
 
 
 @pytest.mark.parametrize("D,P", LONG_D)
