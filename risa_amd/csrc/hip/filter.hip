@@ -807,6 +807,7 @@ __global__ void __launch_bounds__(512) dft_gemm_split_kernel(const float* __rest
 // 2 * 8 * circ_pitch(D) halves.
 // LDS: 40 KB of A tiles + 32 * circ_pitch(D) bytes of copies <= 64 KB
 constexpr int kCircMinD = 128, kCircMaxD = 704;
+constexpr int GC_N = 256;   // circulant kernel: 128 x 256 output tile, a 64 x 64 block per wave
 __host__ __device__ constexpr int circ_pitch(int D) { return ((D + 8 + 7) / 8) * 8 + 8; }
 
 __global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restrict__ X, int M, int D,
@@ -834,8 +835,8 @@ __global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restr
             nb = L % NB;
         }
     }
-    const int m0 = mb * GB_M, n0 = nb * GB_N;
-    const int wm = (wave / 2) * 32, wn = (wave % 2) * 64;
+    const int m0 = mb * GB_M, n0 = nb * GC_N;
+    const int wm = (wave / 4) * 64, wn = (wave % 4) * 64;
     const bool x4 = o.x4 != nullptr;
     const int groups = (o.slices + 3) / 4;
     const int rows = x4 ? groups * o.P * 4 : M;
@@ -863,7 +864,7 @@ __global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restr
     }
     const bool vec = (D % 8) == 0;
 
-    float16_t acc0 = {}, acc1 = {};
+    float16_t acc[2][2] = {};
     auto load = [&](int k0, float (&av)[8]) {
         const int gk = k0 + sc8;
         if (xrow >= 0 && vec && gk + 8 <= D) {
@@ -895,17 +896,31 @@ __global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restr
 #pragma unroll
         for (int kk = 0; kk < GB_K; kk += 16) {
             const int r = lane % 32, kq = kk + (lane / 32) * 8;
-            const half8_t ah = *reinterpret_cast<const half8_t*>(&Ah[buf][wm + r][kq]);
-            const half8_t al = *reinterpret_cast<const half8_t*>(&Al[buf][wm + r][kq]);
+            half8_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                ah[i] = *reinterpret_cast<const half8_t*>(&Ah[buf][wm + 32 * i + r][kq]);
+                al[i] = *reinterpret_cast<const half8_t*>(&Al[buf][wm + 32 * i + r][kq]);
+            }
             const int n = n0 + wn + r, k = k0 + kq;
-            const half8_t bh0 = bfrag(n, k, 0), bl0 = bfrag(n, k, 1);
-            const half8_t bh1 = bfrag(n + 32, k, 0), bl1 = bfrag(n + 32, k, 1);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh1, acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh1, acc1, 0, 0, 0);
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl1, acc1, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                bh[j] = bfrag(n + 32 * j, k, 0);
+                bl[j] = bfrag(n + 32 * j, k, 1);
+            }
+            // per 32 x 32 block the three products in the order of a 32 x 64 wave block
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
         }
     };
     // tiles t = 0 .. T-1 of GB_K; registers r0 hold even tiles, r1 odd ones
@@ -929,14 +944,17 @@ __global__ void __launch_bounds__(512) dft_gemm_circ_kernel(const float* __restr
         __syncthreads();
     }
     // D layout: lane -> column lane%32; value v -> row 8*(v/4) + 4*(lane/32) + v%4
+    const float16_t (&acc2)[2][2] = acc;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const float16_t& acc = j == 0 ? acc0 : acc1;
+        const float16_t& acc = acc2[i][j];
         const int col = n0 + wn + 32 * j + (lane % 32);
         if (col >= D) continue;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int rb = m0 + wm + 8 * q + 4 * (lane / 32);
+            const int rb = m0 + wm + 32 * i + 8 * q + 4 * (lane / 32);
             if (x4) {
                 if (rb >= rows) continue;
                 uint32_t h[4];
@@ -1258,6 +1276,7 @@ static void launch_gemm(risa_filter_t h, const float* in, GemmOut o, long rows, 
     const long out_rows = o.x4 ? (long)((o.slices + 3) / 4) * o.P * 4 : rows;
     dim3 grid(ceil_div(D, GB_N), ceil_div(out_rows, GB_M));
     if (D >= kCircMinD && D <= kCircMaxD) {
+        grid.x = ceil_div(D, GC_N);
         const size_t lds = (size_t)2 * 8 * circ_pitch(D) * sizeof(_Float16);
         if (lds + sizeof(_Float16) * 2 * 2 * GB_M * GB_PITCH > 64 * 1024) fail(RISA_ERR_UNSUPPORTED, "circulant LDS");
         hipLaunchKernelGGL(dft_gemm_circ_kernel, grid, dim3(512), lds, s, in, (int)rows, D, h->ch1.ptr, h->cl1.ptr, sx,
